@@ -1,0 +1,260 @@
+/*
+ * qsv.h -- C ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * This is the drop-in boundary for QPanda's `QuantumGates` gate API
+ * (QPandaSDK/QuantumInstructionHandle/QuantumGates.h:31-474).  QPanda's GPU
+ * back end reaches its CUDA library through dlsym() on `extern "C"` functions
+ * whose parameters are C++ types (QPandaSDK/GPUEmulator/GPUGatesDecl.h:38-75);
+ * this header replaces that table with a true C ABI: an opaque handle, plain
+ * pointers and sizes, `int` status codes.  Each entry point names the
+ * reference function it replaces.
+ *
+ * Conventions (these follow QPanda's CPU simulator, the parity target):
+ *  - amplitude index bit q is qubit q (qubit 0 = least significant bit);
+ *  - complex numbers are interleaved (re, im) pairs of doubles on the host;
+ *  - matrices are row-major complex, U[r][c] = (m[2*(r*d+c)], m[2*(r*d+c)+1]);
+ *  - for a two-qubit matrix on (qa, qb) the row index is 2*bit(qa) + bit(qb);
+ *  - for a k-qubit block on qubits[0..k-1] the row index bit j is bit(qubits[j]);
+ *  - H uses the constant 0.707106781186548 unless `exact` asks for 1/sqrt(2)
+ *    (CPUQuantumGates.cpp:36);
+ *  - RZ is diag(e^{-i theta/2}, e^{+i theta/2}) (CPUQuantumGates.cpp:524-570);
+ *  - iSWAP / sqiSWAP use +i (CPUQuantumGates.cpp:838-899, 978-1039).
+ *
+ * Every call is synchronous with respect to the host unless its comment says
+ * otherwise: readouts wait for the device, gate calls are stream-ordered on the
+ * handle's stream (a readout or qsv_sync() orders them), so callers observe the
+ * reference's synchronous semantics.
+ */
+#ifndef QSV_H
+#define QSV_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (0 = OK, negative = error) ------------------------------ */
+#define QSV_OK              0
+#define QSV_ERR_ARG        -1   /* bad argument (null pointer, bad size)        */
+#define QSV_ERR_QUBIT      -2   /* qubit index out of range or repeated         */
+#define QSV_ERR_ALLOC      -3   /* device or host allocation failed             */
+#define QSV_ERR_DEVICE     -4   /* HIP runtime error                            */
+#define QSV_ERR_PRECISION  -5   /* operation not available at this precision   */
+#define QSV_ERR_UNSUPPORTED -6  /* gate kind / option not supported             */
+#define QSV_ERR_NUMERIC    -7   /* probability of the requested branch is zero  */
+
+/* ---- precisions ------------------------------------------------------------ */
+#define QSV_C128 0   /* complex<double> amplitudes, 16 B each */
+#define QSV_C64  1   /* complex<float> amplitudes, 8 B each; reductions in FP64 */
+
+#define QSV_MAX_QUBITS 34
+
+/* ---- gate records (circuit interchange format, 32 bytes each) -------------
+ * Shared by the engine, the CPU oracle and the reference driver.
+ *   kind        one of QSV_GATE_*
+ *   dagger      nonzero = inverse (the reference's iDagger flag)
+ *   q0,q1,q2    qubits; roles per kind below
+ *   mat         for U1/U2: offset, in complex entries, into the matrix pool
+ *   theta       rotation angle (RX/RY/RZ half-angle convention; CR full angle)
+ * Roles: H/X/RX/RY/RZ/U1: q0 = target.  CNOT: q0 = control, q1 = target.
+ * CZ/CR/ISWAP/SQISWAP: q0, q1.  U2: matrix row = 2*bit(q0)+bit(q1).
+ * TOFFOLI: q0, q1 = controls, q2 = target.  CSWAP: q0 = control, swaps q1/q2.
+ * MEASURE: q0 = qubit, q1 = classical register.  RESET: q0 = qubit.
+ */
+enum {
+    QSV_GATE_H = 1,
+    QSV_GATE_X = 2,
+    QSV_GATE_RX = 3,
+    QSV_GATE_RY = 4,
+    QSV_GATE_RZ = 5,
+    QSV_GATE_CNOT = 6,
+    QSV_GATE_CZ = 7,
+    QSV_GATE_CR = 8,
+    QSV_GATE_ISWAP = 9,
+    QSV_GATE_SQISWAP = 10,
+    QSV_GATE_TOFFOLI = 11,
+    QSV_GATE_CSWAP = 12,
+    QSV_GATE_U1 = 13,
+    QSV_GATE_U2 = 14,
+    QSV_GATE_MEASURE = 15,
+    QSV_GATE_RESET = 16
+};
+
+typedef struct qsv_gate {
+    int32_t kind;
+    int32_t dagger;
+    int32_t q0, q1, q2;
+    int32_t mat;
+    double  theta;
+} qsv_gate;
+
+typedef struct qsv_state qsv_state;   /* opaque device state */
+
+/* ---- lifecycle ------------------------------------------------------------- */
+
+/* Allocate 2^n amplitudes in HBM and set |0...0>.
+ * Replaces CPUQuantumGates::initState (CPUQuantumGates.cpp:1136-1157) and
+ * GATEGPU::initstate (GPUGates.cu:1208-1299).  precision = QSV_C128 | QSV_C64. */
+int qsv_create(int n, int precision, qsv_state **out);
+
+/* Free the state.  Replaces destroyState (GPUQuantumGates.cpp:857-871,
+ * GPUGates.cu:1129-1161). */
+int qsv_destroy(qsv_state *s);
+
+/* |0...0> again without reallocating.  Replaces endGate
+ * (CPUQuantumGates.cpp:1445-1453; GPUGates.cu:1167-1206 clearState). */
+int qsv_reset_zero(qsv_state *s);
+
+int qsv_num_qubits(const qsv_state *s);
+int qsv_precision(const qsv_state *s);
+
+/* Host <-> device copies of the whole state as interleaved (re, im) doubles
+ * (converted to/from FP32 for QSV_C64).  qsv_get_state replaces getState
+ * (GPUGates.cu:2160-2169) / the CPU's mvQuantumStat read-out
+ * (CPUQuantumGates.cpp:64-82).  qsv_set_state has no reference counterpart
+ * (the reference always restarts from |0...0>, QuantumInstructionHandleAPI.cpp:96-100). */
+int qsv_set_state(qsv_state *s, const double *amps);
+int qsv_get_state(qsv_state *s, double *amps);
+
+/* Raw device pointer of the amplitude array (interleaved complex of the state's
+ * precision) and the handle's HIP stream, for callers that keep data in HBM. */
+int qsv_device_ptr(qsv_state *s, void **dptr);
+int qsv_stream(qsv_state *s, void **stream);
+int qsv_sync(qsv_state *s);
+
+/* ---- single-qubit and controlled single-qubit gates -------------------------
+ * ctrl_mask: bit c set = qubit c is a control that must be 1 (0 = uncontrolled).
+ * The target bit must not be in ctrl_mask.  The reference passes controls as a
+ * list with the target appended last (Instruction.cpp:62,155,243,305,415); the
+ * C++ adapter turns that list into this mask. */
+
+/* Hadamard.  Replaces Hadamard(qn,err) (CPUQuantumGates.cpp:92-117) and the
+ * controlled form (CPUQuantumGates.cpp:202-258).  exact != 0 uses 1/sqrt(2). */
+int qsv_h(qsv_state *s, int target, uint64_t ctrl_mask, int exact);
+
+/* Pauli-X / NOT / CNOT / Toffoli / multi-controlled X.  Replaces NOT
+ * (CPUQuantumGates.cpp:1277-1365), CNOT (:668-757), toffoli (:1378-1435). */
+int qsv_x(qsv_state *s, int target, uint64_t ctrl_mask);
+
+/* General 2x2 complex matrix u[8] (row-major re,im).  Serves RX
+ * (CPUQuantumGates.cpp:271-383), RY (:396-511) and a correct
+ * unitarySingleQubitGate (the reference's targets qn-1, :1188). */
+int qsv_apply_1q(qsv_state *s, int target, const double *u, uint64_t ctrl_mask);
+
+/* Diagonal diag(d0, d1), d[4] = {re0, im0, re1, im1}.  Serves RZ
+ * (CPUQuantumGates.cpp:524-655). */
+int qsv_diag(qsv_state *s, int target, const double *d, uint64_t ctrl_mask);
+
+/* Convenience wrappers with the reference's angle conventions. */
+int qsv_rx(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mask);
+int qsv_ry(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mask);
+int qsv_rz(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mask);
+
+/* ---- two- and three-qubit gates ------------------------------------------- */
+
+/* |11> *= (re + i im) on qubits (q1, q2), with extra controls.  CR(theta) is
+ * (cos theta, +-sin theta) (CPUQuantumGates.cpp:773-825); CZ is (-1, 0). */
+int qsv_cphase(qsv_state *s, int q1, int q2, double re, double im, uint64_t ctrl_mask);
+int qsv_cr(qsv_state *s, int q1, int q2, double theta, int dagger);
+
+/* iSWAP / sqiSWAP with the CPU simulator's +i phase (dagger: -i).
+ * Replaces iSWAP (CPUQuantumGates.cpp:838-899), sqiSWAP (:978-1039). */
+int qsv_iswap(qsv_state *s, int q1, int q2, int dagger);
+int qsv_sqiswap(qsv_state *s, int q1, int q2, int dagger, int exact);
+
+/* Fredkin: swap qubits a and b where control c is 1.  Replaces controlSwap
+ * (CPUQuantumGates.cpp:934-965). */
+int qsv_cswap(qsv_state *s, int c, int a, int b);
+
+/* Dense two-qubit unitary u[32] on (qa, qb), row = 2*bit(qa) + bit(qb).
+ * No reference counterpart (the reference has matReverse only,
+ * CPUQuantumGates.cpp:1221-1244). */
+int qsv_apply_2q(qsv_state *s, int qa, int qb, const double *u);
+
+/* Dense k-qubit block (k = 1..5), u = 2^k x 2^k row-major complex doubles.
+ * C128 states apply it in FP64; C64 states use the fused MFMA kernel
+ * (FP32 complex GEMM with FP32 accumulation; split = 1 adds the lo part of
+ * the matrix, U = U_hi + U_lo).  No reference counterpart. */
+int qsv_apply_block(qsv_state *s, const int *qubits, int k, const double *u, int split);
+
+/* ---- readout --------------------------------------------------------------- */
+
+/* Sum of |a|^2 over amplitudes whose bit q is 0 (FP64 on device,
+ * deterministic order).  The quantity qubitMeasure computes
+ * (CPUQuantumGates.cpp:1074-1083; GPUGates.cu:597-682). */
+int qsv_prob0(qsv_state *s, int q, double *p0);
+
+/* Squared norm in FP64. */
+int qsv_norm2(qsv_state *s, double *norm2);
+
+/* Keep branch `outcome` of qubit q, zero the other, scale by `scale`.
+ * (CPUQuantumGates.cpp:1092-1124; GPUGates.cu:684-718) */
+int qsv_collapse(qsv_state *s, int q, int outcome, double scale);
+
+/* Measure qubit q with draw r in [0,1] (the value of the reference's
+ * randGenerator() call): outcome = ((float)r > p0), then collapse and
+ * renormalise by 1/sqrt(p).  Replaces qubitMeasure (CPUQuantumGates.cpp:1071-1126).
+ * When (float)r lies within the rounding band of p0, p0 is recomputed in the
+ * CPU simulator's serial summation order before deciding. */
+int qsv_measure(qsv_state *s, int q, double r, int *outcome);
+
+/* Full marginal distribution over qubits[0..m-1] (m <= 24) into out[2^m];
+ * outcome bit j = bit(qubits[j]) (caller order, CPUQuantumGates.cpp:171-174). */
+int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out);
+
+/* pMeasure compatibility view: the distribution sorted by probability
+ * descending, truncated to 10 entries (CPUQuantumGates.cpp:159-190).
+ * Writes *count entries of (index, probability). */
+int qsv_pmeasure(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count);
+
+/* sum_{i=0}^{n-2} <Z_i Z_{i+1}> in FP64. */
+int qsv_expect_zz_chain(qsv_state *s, double *value);
+
+/* ---- whole circuits ---------------------------------------------------------
+ * Apply gates[0..ngates-1] in order.  mats = matrix pool (complex doubles) for
+ * U1/U2 records.  For C128 each record is one sweep (the per-gate path).
+ * rng: Park-Miller state; each gate consumes one draw exactly as the reference
+ * executor does (QuantumGates.cpp:46-85, e.g. CPUQuantumGates.cpp:97) and
+ * MEASURE consumes one more; NULL skips RNG bookkeeping.  outcomes[creg] receives
+ * measurement results (may be NULL when the circuit has none). */
+typedef struct qsv_run_opts {
+    int exact;          /* 1: H / sqiSWAP use exact 1/sqrt(2)                   */
+    int fuse_k;         /* 0: per-gate path; 2..5: fused blocks of <= k qubits  */
+    int split;          /* fused blocks: 1 = hi+lo split matrix product         */
+    int scheduler;      /* 0: greedy fusion, 1: learned cost model              */
+    int reserved[4];
+} qsv_run_opts;
+
+typedef struct qsv_run_stats {
+    int64_t gates;          /* logical gates applied                     */
+    int64_t launches;       /* kernel launches issued                    */
+    int64_t blocks;         /* fused blocks (0 on the per-gate path)     */
+    int64_t tie_resolutions;/* measurements decided in serial order      */
+    double  device_ms;      /* device time between events around the run */
+    double  model_ms_pred;  /* scheduler's predicted time (fused path)    */
+} qsv_run_stats;
+
+int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngates,
+                    const double *mats, const qsv_run_opts *opts,
+                    int32_t *rng, int *outcomes, qsv_run_stats *stats);
+
+/* ---- Park-Miller "minimal standard" generator (QuantumGates.cpp:46-85) ----- */
+/* Next state from `state` (1..2^31-2); the draw is state / 2147483647.0. */
+int32_t qsv_pm_next(int32_t state);
+/* state after k steps, O(log k) (multiplier 16807^k mod 2^31-1). */
+int32_t qsv_pm_jump(int32_t state, uint64_t k);
+/* The seed the reference derives from a broken-down local time
+ * (QuantumGates.cpp:68-72). */
+int32_t qsv_pm_seed_from_tm(int tm_year, int tm_mon, int tm_mday, int tm_hour, int tm_min, int tm_sec);
+
+/* ---- library info --------------------------------------------------------- */
+const char *qsv_version(void);
+const char *qsv_last_error(void);   /* thread-local message of the last failure */
+int qsv_device_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSV_H */
