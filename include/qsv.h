@@ -212,6 +212,18 @@ int qsv_pmeasure(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *
 /* sum_{i=0}^{n-2} <Z_i Z_{i+1}> in FP64. */
 int qsv_expect_zz_chain(qsv_state *s, double *value);
 
+/* Shot loop of QPandaSDK::run (QPandaAPI.cpp:131-211, QuantumInstructionHandleAPI.cpp:96-114)
+ * without re-simulating the program per shot.  The reference restarts every
+ * shot from |0...0>, runs the program (one draw per gate call) and then measures;
+ * because the gates do not depend on the draws, the pre-measurement state is the
+ * same every shot.  With that state in `s` (left unchanged), each shot skips
+ * `gate_draws` draws by jump-ahead and measures qubits[0..m-1] in order with the
+ * reference's decision rule; bit j of out[shot] is the outcome of qubits[j].
+ * *rng is the generator state before the first shot's gates and is advanced past
+ * the last shot.  *ties (may be NULL) counts decisions re-made in serial order. */
+int qsv_sample_shots(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
+                     int64_t shots, uint64_t *out, int64_t *ties);
+
 /* ---- whole circuits ---------------------------------------------------------
  * Apply gates[0..ngates-1] in order.  mats = matrix pool (complex doubles) for
  * U1/U2 records.  For C128 each record is one sweep (the per-gate path).

@@ -1,0 +1,22 @@
+#!/bin/bash
+# Builds qpanda_amd/libqsv.so (HIP kernels + C ABI) for gfx950.
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+cd "$HERE"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops
+       -Wall -Wno-unused-function -I"$HERE/../include")
+mkdir -p _build
+OBJS=()
+for src in csrc/*.hip csrc/*.cpp; do
+  obj="_build/$(basename "$src").o"
+  if [[ "$src" == *.hip ]]; then
+    "$HIPCC" "${FLAGS[@]}" -c "$src" -o "$obj"
+  else
+    "$HIPCC" "${FLAGS[@]}" -x hip -c "$src" -o "$obj"
+  fi
+  OBJS+=("$obj")
+done
+"$HIPCC" -shared -fPIC --offload-arch=gfx950 -o libqsv.so.tmp "${OBJS[@]}"
+mv libqsv.so.tmp libqsv.so
+echo "built $HERE/libqsv.so"

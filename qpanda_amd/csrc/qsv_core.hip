@@ -1,0 +1,508 @@
+// qsv_core.hip -- the C ABI of include/qsv.h: state lifecycle, gate entry
+// points, readouts, the Park-Miller generator and the per-gate circuit runner.
+//
+// Every entry point validates its qubits on the host before any launch, so no
+// kernel ever receives an index outside the state.
+#include "qsv_internal.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace qsv {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &m) { g_last_error = m; }
+
+int fail(int code, const std::string &m)
+{
+    g_last_error = m;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return QSV_ERR_DEVICE;
+}
+
+size_t amp_bytes(const qsv_state *s) { return s->prec == QSV_C128 ? 16 : 8; }
+
+int check_qubit(const qsv_state *s, int q)
+{
+    if (q < 0 || q >= s->n) return fail(QSV_ERR_QUBIT, "qubit index out of range");
+    return QSV_OK;
+}
+
+int ensure_aux(qsv_state *s, size_t bytes)
+{
+    if (s->aux_bytes >= bytes) return QSV_OK;
+    if (s->aux) {
+        QSV_HIP(hipStreamSynchronize(s->stream));
+        QSV_HIP(hipFree(s->aux));
+        s->aux = nullptr;
+        s->aux_bytes = 0;
+    }
+    if (hipMalloc(&s->aux, bytes) != hipSuccess) {
+        s->aux = nullptr;
+        return fail(QSV_ERR_ALLOC, "cannot allocate readout scratch");
+    }
+    s->aux_bytes = bytes;
+    return QSV_OK;
+}
+
+// Controls must be inside the state and must not include the target bits.
+static int check_mask(const qsv_state *s, uint64_t ctrl, uint64_t targets)
+{
+    if (s->n < 64 && (ctrl >> s->n)) return fail(QSV_ERR_QUBIT, "control qubit out of range");
+    if (ctrl & targets) return fail(QSV_ERR_QUBIT, "control overlaps target");
+    return QSV_OK;
+}
+
+}  // namespace qsv
+
+using namespace qsv;
+
+#define CHECK_STATE(s) \
+    do { if (!(s) || !(s)->amps) return fail(QSV_ERR_ARG, "null state"); } while (0)
+
+extern "C" {
+
+const char *qsv_version(void) { return "qsv 0.1 gfx950"; }
+const char *qsv_last_error(void) { return g_last_error.c_str(); }
+
+int qsv_device_count(int *count)
+{
+    if (!count) return fail(QSV_ERR_ARG, "null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return QSV_OK;
+}
+
+int qsv_create(int n, int precision, qsv_state **out)
+{
+    if (!out) return fail(QSV_ERR_ARG, "null out");
+    *out = nullptr;
+    if (n < 1 || n > QSV_MAX_QUBITS) return fail(QSV_ERR_QUBIT, "qubit count out of range");
+    if (precision != QSV_C128 && precision != QSV_C64) return fail(QSV_ERR_PRECISION, "bad precision");
+    qsv_state *s = new qsv_state();
+    s->n = n;
+    s->prec = precision;
+    s->dim = uint64_t(1) << n;
+    int rc = QSV_OK;
+    if (hipGetDevice(&s->device) != hipSuccess) rc = fail(QSV_ERR_DEVICE, "no HIP device");
+    if (!rc && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(QSV_ERR_DEVICE, "stream creation failed");
+    if (!rc && hipMalloc(&s->amps, s->dim * amp_bytes(s)) != hipSuccess) {
+        s->amps = nullptr;
+        rc = fail(QSV_ERR_ALLOC, "cannot allocate the state in HBM");
+    }
+    if (!rc && hipMalloc((void **)&s->red, sizeof(double) * kRedSlots) != hipSuccess) {
+        s->red = nullptr;
+        rc = fail(QSV_ERR_ALLOC, "cannot allocate reduction scratch");
+    }
+    if (!rc && hipHostMalloc((void **)&s->host_red, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) {
+        s->host_red = nullptr;
+        rc = fail(QSV_ERR_ALLOC, "cannot allocate pinned host buffer");
+    }
+    if (!rc) rc = launch_fill_zero(s);
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(QSV_ERR_DEVICE, "init failed");
+    if (rc) {
+        if (s->amps) (void)hipFree(s->amps);
+        if (s->red) (void)hipFree(s->red);
+        if (s->host_red) (void)hipHostFree(s->host_red);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return QSV_OK;
+}
+
+int qsv_destroy(qsv_state *s)
+{
+    if (!s) return QSV_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->amps) (void)hipFree(s->amps);
+    if (s->red) (void)hipFree(s->red);
+    if (s->aux) (void)hipFree(s->aux);
+    if (s->host_red) (void)hipHostFree(s->host_red);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return QSV_OK;
+}
+
+int qsv_reset_zero(qsv_state *s)
+{
+    CHECK_STATE(s);
+    return launch_fill_zero(s);
+}
+
+int qsv_num_qubits(const qsv_state *s) { return s ? s->n : -1; }
+int qsv_precision(const qsv_state *s) { return s ? s->prec : -1; }
+
+int qsv_set_state(qsv_state *s, const double *amps)
+{
+    CHECK_STATE(s);
+    if (!amps) return fail(QSV_ERR_ARG, "null amps");
+    if (s->prec == QSV_C128) {
+        QSV_HIP(hipMemcpyAsync(s->amps, amps, s->dim * 16, hipMemcpyHostToDevice, s->stream));
+    } else {
+        std::vector<float> tmp(2 * s->dim);
+        for (uint64_t i = 0; i < 2 * s->dim; i++) tmp[i] = (float)amps[i];
+        QSV_HIP(hipMemcpyAsync(s->amps, tmp.data(), s->dim * 8, hipMemcpyHostToDevice, s->stream));
+        QSV_HIP(hipStreamSynchronize(s->stream));
+    }
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    return QSV_OK;
+}
+
+int qsv_get_state(qsv_state *s, double *amps)
+{
+    CHECK_STATE(s);
+    if (!amps) return fail(QSV_ERR_ARG, "null amps");
+    if (s->prec == QSV_C128) {
+        QSV_HIP(hipMemcpyAsync(amps, s->amps, s->dim * 16, hipMemcpyDeviceToHost, s->stream));
+        QSV_HIP(hipStreamSynchronize(s->stream));
+    } else {
+        std::vector<float> tmp(2 * s->dim);
+        QSV_HIP(hipMemcpyAsync(tmp.data(), s->amps, s->dim * 8, hipMemcpyDeviceToHost, s->stream));
+        QSV_HIP(hipStreamSynchronize(s->stream));
+        for (uint64_t i = 0; i < 2 * s->dim; i++) amps[i] = tmp[i];
+    }
+    return QSV_OK;
+}
+
+int qsv_device_ptr(qsv_state *s, void **dptr)
+{
+    CHECK_STATE(s);
+    if (!dptr) return fail(QSV_ERR_ARG, "null");
+    *dptr = s->amps;
+    return QSV_OK;
+}
+
+int qsv_stream(qsv_state *s, void **stream)
+{
+    CHECK_STATE(s);
+    if (!stream) return fail(QSV_ERR_ARG, "null");
+    *stream = (void *)s->stream;
+    return QSV_OK;
+}
+
+int qsv_sync(qsv_state *s)
+{
+    CHECK_STATE(s);
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    return QSV_OK;
+}
+
+// ---------------------------------------------------------------- gates ----
+static const double kSq2Ref = 0.707106781186548;     // CPUQuantumGates.cpp:36
+static const double kSq2Exact = 0.70710678118654752440;
+
+static int one_qubit(qsv_state *s, int op, int t, uint64_t ctrl, const double *u, double h)
+{
+    CHECK_STATE(s);
+    int rc = check_qubit(s, t);
+    if (rc) return rc;
+    const uint64_t tb = uint64_t(1) << t;
+    if ((rc = check_mask(s, ctrl, tb))) return rc;
+    InsertSet ins = insert_set_from_mask(ctrl | tb);
+    return launch_pair(s, op, ins, ctrl, ctrl | tb, u, h);
+}
+
+int qsv_h(qsv_state *s, int target, uint64_t ctrl_mask, int exact)
+{
+    return one_qubit(s, OP_H, target, ctrl_mask, nullptr, exact ? kSq2Exact : kSq2Ref);
+}
+
+int qsv_x(qsv_state *s, int target, uint64_t ctrl_mask)
+{
+    return one_qubit(s, OP_X, target, ctrl_mask, nullptr, 0);
+}
+
+int qsv_apply_1q(qsv_state *s, int target, const double *u, uint64_t ctrl_mask)
+{
+    if (!u) return fail(QSV_ERR_ARG, "null matrix");
+    return one_qubit(s, OP_U, target, ctrl_mask, u, 0);
+}
+
+int qsv_diag(qsv_state *s, int target, const double *d, uint64_t ctrl_mask)
+{
+    if (!d) return fail(QSV_ERR_ARG, "null diagonal");
+    return one_qubit(s, OP_DIAG, target, ctrl_mask, d, 0);
+}
+
+// RX: [[c, -i s], [-i s, c]], dagger +i (CPUQuantumGates.cpp:271-309).
+int qsv_rx(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mask)
+{
+    const double c = std::cos(theta / 2), sn = std::sin(theta / 2);
+    const double k = dagger ? 1.0 : -1.0;
+    const double u[8] = {c, 0, 0, k * sn, 0, k * sn, c, 0};
+    return qsv_apply_1q(s, target, u, ctrl_mask);
+}
+
+// RY: [[c, -s], [s, c]], dagger transposed (CPUQuantumGates.cpp:473-511).
+int qsv_ry(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mask)
+{
+    const double c = std::cos(theta / 2), sn = std::sin(theta / 2);
+    const double g = dagger ? -1.0 : 1.0;
+    const double u[8] = {c, 0, -g * sn, 0, g * sn, 0, c, 0};
+    return qsv_apply_1q(s, target, u, ctrl_mask);
+}
+
+// RZ: diag(e^{-i t/2}, e^{+i t/2}), dagger conjugated (CPUQuantumGates.cpp:524-570).
+int qsv_rz(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mask)
+{
+    const double c = std::cos(theta / 2), sn = std::sin(theta / 2);
+    const double s1 = dagger ? -sn : sn;
+    const double d[4] = {c, -s1, c, s1};
+    return qsv_diag(s, target, d, ctrl_mask);
+}
+
+int qsv_cphase(qsv_state *s, int q1, int q2, double re, double im, uint64_t ctrl_mask)
+{
+    CHECK_STATE(s);
+    int rc;
+    if ((rc = check_qubit(s, q1)) || (rc = check_qubit(s, q2))) return rc;
+    if (q1 == q2) return QSV_OK;            // the reference's loop is empty (CPUQuantumGates.cpp:813)
+    const uint64_t m = (uint64_t(1) << q1) | (uint64_t(1) << q2);
+    if ((rc = check_mask(s, ctrl_mask, m))) return rc;
+    InsertSet ins = insert_set_from_mask(m | ctrl_mask);
+    return launch_phase(s, ins, m | ctrl_mask, re, im);
+}
+
+int qsv_cr(qsv_state *s, int q1, int q2, double theta, int dagger)
+{
+    const double c = std::cos(theta), sn = std::sin(theta);
+    return qsv_cphase(s, q1, q2, c, dagger ? -sn : sn, 0);
+}
+
+static int iswap_like(qsv_state *s, int q1, int q2, const double *u)
+{
+    CHECK_STATE(s);
+    int rc;
+    if ((rc = check_qubit(s, q1)) || (rc = check_qubit(s, q2))) return rc;
+    if (q1 == q2) return QSV_OK;            // empty loop in the reference
+    const int hi = std::max(q1, q2), lo = std::min(q1, q2);
+    InsertSet ins = insert_set_from_mask((uint64_t(1) << hi) | (uint64_t(1) << lo));
+    // member 0 = |hi=0, lo=1>, member 1 = |hi=1, lo=0> (CPUQuantumGates.cpp:877-895)
+    return launch_pair(s, OP_U, ins, uint64_t(1) << lo, uint64_t(1) << hi, u, 0);
+}
+
+int qsv_iswap(qsv_state *s, int q1, int q2, int dagger)
+{
+    const double k = dagger ? -1.0 : 1.0;
+    const double u[8] = {0, 0, 0, k, 0, k, 0, 0};
+    return iswap_like(s, q1, q2, u);
+}
+
+int qsv_sqiswap(qsv_state *s, int q1, int q2, int dagger, int exact)
+{
+    const double h = exact ? kSq2Exact : kSq2Ref, k = dagger ? -h : h;
+    const double u[8] = {h, 0, 0, k, 0, k, h, 0};
+    return iswap_like(s, q1, q2, u);
+}
+
+int qsv_cswap(qsv_state *s, int c, int a, int b)
+{
+    CHECK_STATE(s);
+    int rc;
+    if ((rc = check_qubit(s, c)) || (rc = check_qubit(s, a)) || (rc = check_qubit(s, b))) return rc;
+    if (c == a || c == b || a == b) return fail(QSV_ERR_QUBIT, "controlSwap qubits must differ");
+    const uint64_t bc = uint64_t(1) << c, ba = uint64_t(1) << a, bb = uint64_t(1) << b;
+    InsertSet ins = insert_set_from_mask(bc | ba | bb);
+    return launch_pair(s, OP_X, ins, bc | ba, bc | bb, nullptr, 0);
+}
+
+int qsv_apply_2q(qsv_state *s, int qa, int qb, const double *u)
+{
+    CHECK_STATE(s);
+    if (!u) return fail(QSV_ERR_ARG, "null matrix");
+    int rc;
+    if ((rc = check_qubit(s, qa)) || (rc = check_qubit(s, qb))) return rc;
+    if (qa == qb) return fail(QSV_ERR_QUBIT, "two-qubit gate on one qubit");
+    const int qs[2] = {qb, qa};              // block bit 0 = qb, bit 1 = qa
+    return launch_block(s, qs, 2, u, 0);
+}
+
+int qsv_apply_block(qsv_state *s, const int *qubits, int k, const double *u, int split)
+{
+    CHECK_STATE(s);
+    if (!qubits || !u) return fail(QSV_ERR_ARG, "null argument");
+    if (k < 1 || k > 5 || k > s->n) return fail(QSV_ERR_ARG, "block size must be 1..5");
+    uint64_t seen = 0;
+    for (int j = 0; j < k; j++) {
+        int rc = check_qubit(s, qubits[j]);
+        if (rc) return rc;
+        if (seen >> qubits[j] & 1) return fail(QSV_ERR_QUBIT, "repeated qubit in block");
+        seen |= uint64_t(1) << qubits[j];
+    }
+    if (s->prec == QSV_C64 && k >= 2) return launch_block_mfma(s, qubits, k, u, split);
+    return launch_block(s, qubits, k, u, 0);
+}
+
+// -------------------------------------------------------------- readout ----
+int qsv_prob0(qsv_state *s, int q, double *p0)
+{
+    CHECK_STATE(s);
+    if (!p0) return fail(QSV_ERR_ARG, "null");
+    int rc = check_qubit(s, q);
+    if (rc) return rc;
+    return reduce_prob0(s, q, p0);
+}
+
+int qsv_norm2(qsv_state *s, double *v)
+{
+    CHECK_STATE(s);
+    if (!v) return fail(QSV_ERR_ARG, "null");
+    return reduce_norm2(s, v);
+}
+
+int qsv_expect_zz_chain(qsv_state *s, double *v)
+{
+    CHECK_STATE(s);
+    if (!v) return fail(QSV_ERR_ARG, "null");
+    return reduce_zz_chain(s, v);
+}
+
+int qsv_collapse(qsv_state *s, int q, int outcome, double scale)
+{
+    CHECK_STATE(s);
+    int rc = check_qubit(s, q);
+    if (rc) return rc;
+    return launch_collapse(s, q, outcome ? 1 : 0, scale);
+}
+
+// Largest state for which a tie is re-decided by the serial-order sum
+// (2^23 sequential terms on one lane).
+static const int kSerialTieMaxQubits = 24;
+
+int qsv_measure(qsv_state *s, int q, double r, int *outcome)
+{
+    CHECK_STATE(s);
+    if (!outcome) return fail(QSV_ERR_ARG, "null");
+    int rc = check_qubit(s, q);
+    if (rc) return rc;
+    double p0;
+    if ((rc = reduce_prob0(s, q, &p0))) return rc;
+    const float fr = (float)r;                       // CPUQuantumGates.cpp:1086
+    // The CPU simulator's serial sum can differ from this one by at most
+    // about 2^n rounding steps; inside that band decide with its own order.
+    const double band = std::ldexp(DBL_EPSILON, s->n + 1) * std::max(1.0, (double)fr);
+    if (std::fabs((double)fr - p0) <= band && s->n <= kSerialTieMaxQubits) {
+        if ((rc = serial_prob0(s, q, &p0))) return rc;
+        s->tie_resolutions++;
+    }
+    const int out = ((double)fr > p0) ? 1 : 0;       // :1088
+    const double scale = out ? 1 / std::sqrt(1 - p0) : 1 / std::sqrt(p0);   // :1096, :1111
+    if ((rc = launch_collapse(s, q, out, scale))) return rc;
+    *outcome = out;
+    return QSV_OK;
+}
+
+int qsv_sample_shots(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
+                     int64_t shots, uint64_t *out, int64_t *ties)
+{
+    CHECK_STATE(s);
+    if (!qubits || !rng || !out || shots < 0 || gate_draws < 0) return fail(QSV_ERR_ARG, "bad argument");
+    if (m < 1 || m > 64) return fail(QSV_ERR_ARG, "1..64 measured qubits");
+    for (int j = 0; j < m; j++) {
+        int rc = check_qubit(s, qubits[j]);
+        if (rc) return rc;
+    }
+    void *work = nullptr;
+    const size_t bytes = s->dim * amp_bytes(s);
+    if (hipMalloc(&work, bytes) != hipSuccess) return fail(QSV_ERR_ALLOC, "cannot allocate shot scratch");
+    qsv_state w = *s;                 // same stream and scratch, different amplitudes
+    w.amps = work;
+    int rc = QSV_OK;
+    const int64_t ties0 = s->tie_resolutions;
+    for (int64_t sh = 0; sh < shots && !rc; sh++) {
+        hipError_t e = hipMemcpyAsync(work, s->amps, bytes, hipMemcpyDeviceToDevice, s->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpyAsync"); break; }
+        *rng = qsv_pm_jump(*rng, (uint64_t)gate_draws);
+        uint64_t bits = 0;
+        for (int j = 0; j < m && !rc; j++) {
+            *rng = qsv_pm_next(*rng);
+            int o = 0;
+            rc = qsv_measure(&w, qubits[j], (double)*rng / 2147483647, &o);
+            bits |= (uint64_t)o << j;
+        }
+        out[sh] = bits;
+    }
+    s->launches = w.launches;
+    s->tie_resolutions = w.tie_resolutions;
+    if (ties) *ties = s->tie_resolutions - ties0;
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(work);
+    return rc;
+}
+
+int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out)
+{
+    CHECK_STATE(s);
+    if (!qubits || !out) return fail(QSV_ERR_ARG, "null");
+    if (m < 1 || m > 24 || m > s->n) return fail(QSV_ERR_ARG, "marginal over 1..24 qubits");
+    uint64_t seen = 0;
+    for (int j = 0; j < m; j++) {
+        int rc = check_qubit(s, qubits[j]);
+        if (rc) return rc;
+        if (seen >> qubits[j] & 1) return fail(QSV_ERR_QUBIT, "repeated qubit");
+        seen |= uint64_t(1) << qubits[j];
+    }
+    return marginals(s, qubits, m, out);
+}
+
+int qsv_pmeasure(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count)
+{
+    if (!idx || !prob || !count) return fail(QSV_ERR_ARG, "null");
+    std::vector<double> full((size_t)1 << std::max(0, std::min(m, 24)));
+    int rc = qsv_marginals(s, qubits, m, full.data());
+    if (rc) return rc;
+    std::vector<std::pair<uint64_t, double>> v(full.size());
+    for (size_t i = 0; i < full.size(); i++) v[i] = {i, full[i]};
+    // CPUQuantumGates.cpp:178-187: sort by probability descending, keep 10.
+    std::sort(v.begin(), v.end(), [](const std::pair<uint64_t, double> &a, const std::pair<uint64_t, double> &b) {
+        return a.second > b.second;
+    });
+    const int keep = (int)std::min<size_t>(10, v.size());
+    for (int i = 0; i < keep; i++) {
+        idx[i] = v[i].first;
+        prob[i] = v[i].second;
+    }
+    *count = keep;
+    return QSV_OK;
+}
+
+// ------------------------------------------------------------------ RNG ----
+int32_t qsv_pm_next(int32_t st)
+{
+    const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+    int32_t t = ia * (st % iq) - ir * (st / iq);
+    if (t < 0) t += im;
+    return t;
+}
+
+int32_t qsv_pm_jump(int32_t st, uint64_t k)
+{
+    uint64_t base = 16807, acc = 1;
+    const uint64_t m = 2147483647ull;
+    while (k) {
+        if (k & 1) acc = acc * base % m;
+        base = base * base % m;
+        k >>= 1;
+    }
+    return (int32_t)((uint64_t)st * acc % m);
+}
+
+int32_t qsv_pm_seed_from_tm(int y, int mo, int d, int h, int mi, int se)
+{
+    return y + 70 * (mo + 1 + 12 * (d + 31 * (h + 23 * (mi + 59 * se))));
+}
+
+}  // extern "C"

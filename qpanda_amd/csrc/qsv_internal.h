@@ -1,0 +1,115 @@
+// qsv_internal.h -- engine internals shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qsv.h"
+
+namespace qsv {
+
+constexpr int kMaxIns = 40;          // zero-bit insert positions per launch
+constexpr int kBlock = 256;          // threads per workgroup for sweeps
+
+// Interleaved complex amplitude types (16 B and 8 B).
+using c128 = double2;
+using c64 = float2;
+
+// Zero-bit insertion: given a compact index i over the free bits, place a 0
+// at each sorted position.  Positions are in final-index coordinates and
+// ascending, so each step shifts the high part left by one.
+struct InsertSet {
+    int n;                 // number of positions
+    int pos[kMaxIns];      // ascending
+};
+
+__host__ __device__ inline uint64_t expand_index(uint64_t i, const InsertSet &s)
+{
+    for (int j = 0; j < s.n; j++) {
+        const int p = s.pos[j];
+        const uint64_t lo = i & ((uint64_t(1) << p) - 1);
+        i = ((i >> p) << (p + 1)) | lo;
+    }
+    return i;
+}
+
+// Scalar 2x2 complex matrix in double precision (row-major).
+struct Mat2 {
+    double m[8];
+};
+
+struct qsv_error_state {
+    std::string msg;
+};
+
+void set_error(const std::string &m);
+int fail(int code, const std::string &m);
+int hip_fail(hipError_t e, const char *what);
+
+#define QSV_HIP(call)                                              \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return ::qsv::hip_fail(e_, #call);   \
+    } while (0)
+
+}  // namespace qsv
+
+struct qsv_state {
+    int n = 0;
+    int prec = QSV_C128;
+    uint64_t dim = 0;
+    void *amps = nullptr;            // device, interleaved complex
+    hipStream_t stream = nullptr;
+    double *red = nullptr;           // device reduction scratch (kRedSlots doubles)
+    double *host_red = nullptr;      // pinned host mirror
+    void *aux = nullptr;             // device scratch for readouts (marginal partials)
+    size_t aux_bytes = 0;
+    double *mats = nullptr;          // device copy of a circuit's matrix pool
+    size_t mats_bytes = 0;
+    int device = 0;
+    int64_t launches = 0;
+    int64_t tie_resolutions = 0;
+};
+
+namespace qsv {
+
+constexpr int kRedSlots = 8192;      // partial sums per reduction (device scratch)
+
+size_t amp_bytes(const qsv_state *s);
+int check_qubit(const qsv_state *s, int q);
+int ensure_aux(qsv_state *s, size_t bytes);
+
+// Gate launchers (qsv_gates.hip).  ops for the pair kernel:
+enum PairOp { OP_H = 0, OP_X = 1, OP_U = 2, OP_DIAG = 3 };
+int launch_pair(qsv_state *s, int op, const InsertSet &ins, uint64_t or0, uint64_t or1,
+                const double *u, double hscale);
+int launch_phase(qsv_state *s, const InsertSet &ins, uint64_t orm, double re, double im);
+int launch_block(qsv_state *s, const int *qubits, int k, const double *u, uint64_t ctrl_mask);
+int launch_block_dev(qsv_state *s, const int *qubits, int k, const double *u_dev, uint64_t ctrl_mask);
+int launch_fill_zero(qsv_state *s);
+
+// Reductions (qsv_readout.hip).
+int reduce_prob0(qsv_state *s, int q, double *p0);
+int reduce_norm2(qsv_state *s, double *v);
+int reduce_zz_chain(qsv_state *s, double *v);
+int serial_prob0(qsv_state *s, int q, double *p0);
+int launch_collapse(qsv_state *s, int q, int outcome, double scale);
+int marginals(qsv_state *s, const int *qubits, int m, double *out);
+
+// Fused FP32 MFMA block (qsv_fused.hip).
+int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int split);
+
+// Build an insert set from a bit mask (ascending positions).
+inline InsertSet insert_set_from_mask(uint64_t mask)
+{
+    InsertSet s{};
+    s.n = 0;
+    for (int b = 0; b < 64; b++)
+        if (mask >> b & 1) s.pos[s.n++] = b;
+    return s;
+}
+
+}  // namespace qsv

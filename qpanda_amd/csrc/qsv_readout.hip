@@ -1,0 +1,302 @@
+// qsv_readout.hip -- FP64 reductions, collapse and marginal distributions.
+//
+// All sums are deterministic: the partition of the index range into
+// workgroups, each thread's stride, the wave butterfly (xor shuffles) and the
+// final pass are fixed functions of the state size, so a readout is
+// bit-identical from run to run.  Reductions accumulate in FP64 for both
+// precisions.  QPanda's CUDA counterpart is a shared-memory tree per block
+// followed by one thread summing the partials serially
+// (GPUGates.cu:597-682); its CPU simulator sums serially in index order
+// (CPUQuantumGates.cpp:1077-1083), which serial_prob0 reproduces for the
+// measurement tie band.
+#include "qsv_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace qsv {
+
+namespace {
+
+enum RedMode { RED_NORM = 0, RED_PROB0 = 1, RED_ZZ = 2 };
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum with a fixed order: wave butterflies, then wave 0 adds the four
+// wave totals in index order.
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBlock / 64; w++) t += lds[w];
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ double norm_of(T v)
+{
+    const double x = v.x, y = v.y;
+    return x * x + y * y;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_reduce(const T *__restrict__ a, uint64_t count, uint64_t chunk,
+                                                    InsertSet ins, int nq, double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / 64];
+    const uint64_t start = (uint64_t)blockIdx.x * chunk;
+    uint64_t end = start + chunk;
+    if (end > count) end = count;
+    const uint64_t zmask = nq > 1 ? ((uint64_t(1) << (nq - 1)) - 1) : 0;
+    double acc = 0;
+    for (uint64_t i = start + threadIdx.x; i < end; i += kBlock) {
+        if (MODE == RED_PROB0) {
+            acc += norm_of(a[expand_index(i, ins)]);
+        } else if (MODE == RED_ZZ) {
+            const int flips = __popcll((i ^ (i >> 1)) & zmask);
+            acc += norm_of(a[i]) * (double)(nq - 1 - 2 * flips);
+        } else {
+            acc += norm_of(a[i]);
+        }
+    }
+    const double t = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ partial, int n,
+                                                          double *__restrict__ out)
+{
+    __shared__ double lds[kBlock / 64];
+    double acc = 0;
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += partial[i];
+    const double t = block_sum(acc, lds);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// CPU-simulator order: for each block of 2*st, for j in the lower half,
+// p += hypot(re, im)^2, every operation rounded separately.
+template <typename T>
+__global__ void k_serial_prob0(const T *__restrict__ a, uint64_t dim, uint64_t st, double *out)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double p = 0;
+    for (uint64_t i = 0; i < dim; i += 2 * st)
+        for (uint64_t j = i; j < i + st; j++) {
+            const double h = hypot((double)a[j].x, (double)a[j].y);
+            p = p + h * h;
+        }
+    *out = p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_collapse(T *__restrict__ a, uint64_t npairs, InsertSet ins,
+                                                      uint64_t bit, int outcome, double scale)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npairs) return;
+    const uint64_t i0 = expand_index(p, ins), i1 = i0 | bit;
+    const uint64_t keep = outcome ? i1 : i0, drop = outcome ? i0 : i1;
+    using R = decltype(T{}.x);
+    T v = a[keep], z;
+    z.x = 0; z.y = 0;
+    v.x = (R)((double)v.x * scale);
+    v.y = (R)((double)v.y * scale);
+    a[keep] = v;
+    a[drop] = z;
+}
+
+// ---- marginals --------------------------------------------------------------
+constexpr int kTileBits = 12;
+
+// Stage 1: one tile of 2^tb consecutive amplitudes per workgroup; probabilities
+// into LDS, then pairwise sums over every unmeasured tile bit in ascending bit
+// order.  Writes 2^ml sums (ml = measured bits inside the tile).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_marg_tile(const T *__restrict__ a, int tb, uint64_t meas_low_mask,
+                                                       int ml, InsertSet low_pos, double *__restrict__ partial)
+{
+    __shared__ double p[1 << kTileBits];
+    const int tsize = 1 << tb;
+    const uint64_t base = (uint64_t)blockIdx.x << tb;
+    for (int i = threadIdx.x; i < tsize; i += kBlock) p[i] = norm_of(a[base + i]);
+    __syncthreads();
+    int done = 0;
+    for (int u = 0; u < tb; u++) {
+        if (meas_low_mask >> u & 1) continue;
+        const int ub = 1 << u;
+        for (int i = threadIdx.x; i < tsize; i += kBlock)
+            if ((i & (done | ub)) == 0) p[i] += p[i | ub];
+        done |= ub;
+        __syncthreads();
+    }
+    const int nb = 1 << ml;
+    for (int lb = threadIdx.x; lb < nb; lb += kBlock) {
+        int idx = 0;
+        for (int r = 0; r < ml; r++)
+            if (lb >> r & 1) idx |= 1 << low_pos.pos[r];
+        partial[(uint64_t)blockIdx.x * nb + lb] = p[idx];
+    }
+}
+
+struct MargMap {
+    int m;
+    int ml;                  // measured bits inside a tile
+    int low_rank[24];        // caller bit j -> rank among low positions, or -1
+    int high_shift[24];      // caller bit j -> tile-index bit, for high positions
+};
+
+// Stage 2: output o sums the tile partials of every tile whose measured high
+// bits match o, in ascending tile order (strided per thread, then the fixed
+// block sum).
+__global__ __launch_bounds__(kBlock) void k_marg_final(const double *__restrict__ partial, MargMap mm,
+                                                        InsertSet high_ins, uint64_t ntiles_each,
+                                                        double *__restrict__ out)
+{
+    __shared__ double lds[kBlock / 64];
+    const uint64_t o = blockIdx.x;
+    int lb = 0;
+    uint64_t hp = 0;
+    for (int j = 0; j < mm.m; j++) {
+        const uint64_t v = (o >> j) & 1;
+        if (mm.low_rank[j] >= 0) lb |= (int)v << mm.low_rank[j];
+        else hp |= v << mm.high_shift[j];
+    }
+    const int nb = 1 << mm.ml;
+    double acc = 0;
+    for (uint64_t t = threadIdx.x; t < ntiles_each; t += kBlock) {
+        const uint64_t tile = expand_index(t, high_ins) | hp;
+        acc += partial[tile * nb + lb];
+    }
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) out[o] = s;
+}
+
+int reduce_common(qsv_state *s, int mode, int q, double *result)
+{
+    InsertSet ins{};
+    uint64_t count = s->dim;
+    if (mode == RED_PROB0) {
+        ins.n = 1;
+        ins.pos[0] = q;
+        count = s->dim >> 1;
+    }
+    uint64_t nblk = (count + (uint64_t)kBlock * 16 - 1) / ((uint64_t)kBlock * 16);
+    if (nblk > 4096) nblk = 4096;
+    if (nblk == 0) nblk = 1;
+    uint64_t chunk = (count + nblk - 1) / nblk;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    nblk = (count + chunk - 1) / chunk;
+    double *partial = s->red;
+    double *out = s->red + (kRedSlots - 1);
+    if (s->prec == QSV_C128) {
+        const c128 *a = (const c128 *)s->amps;
+        if (mode == RED_PROB0) hipLaunchKernelGGL((k_reduce<c128, RED_PROB0>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
+        else if (mode == RED_ZZ) hipLaunchKernelGGL((k_reduce<c128, RED_ZZ>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
+        else hipLaunchKernelGGL((k_reduce<c128, RED_NORM>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
+    } else {
+        const c64 *a = (const c64 *)s->amps;
+        if (mode == RED_PROB0) hipLaunchKernelGGL((k_reduce<c64, RED_PROB0>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
+        else if (mode == RED_ZZ) hipLaunchKernelGGL((k_reduce<c64, RED_ZZ>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
+        else hipLaunchKernelGGL((k_reduce<c64, RED_NORM>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
+    }
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s->stream, partial, (int)nblk, out);
+    s->launches += 2;
+    QSV_HIP(hipGetLastError());
+    QSV_HIP(hipMemcpyAsync(s->host_red, out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    *result = s->host_red[0];
+    return QSV_OK;
+}
+
+}  // namespace
+
+int reduce_prob0(qsv_state *s, int q, double *p0) { return reduce_common(s, RED_PROB0, q, p0); }
+int reduce_norm2(qsv_state *s, double *v) { return reduce_common(s, RED_NORM, 0, v); }
+int reduce_zz_chain(qsv_state *s, double *v) { return reduce_common(s, RED_ZZ, 0, v); }
+
+int serial_prob0(qsv_state *s, int q, double *p0)
+{
+    double *out = s->red + (kRedSlots - 2);
+    const uint64_t st = uint64_t(1) << q;
+    if (s->prec == QSV_C128)
+        hipLaunchKernelGGL(k_serial_prob0<c128>, dim3(1), dim3(64), 0, s->stream, (const c128 *)s->amps, s->dim, st, out);
+    else
+        hipLaunchKernelGGL(k_serial_prob0<c64>, dim3(1), dim3(64), 0, s->stream, (const c64 *)s->amps, s->dim, st, out);
+    s->launches++;
+    QSV_HIP(hipGetLastError());
+    QSV_HIP(hipMemcpyAsync(s->host_red, out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    *p0 = s->host_red[0];
+    return QSV_OK;
+}
+
+int launch_collapse(qsv_state *s, int q, int outcome, double scale)
+{
+    InsertSet ins{};
+    ins.n = 1;
+    ins.pos[0] = q;
+    const uint64_t npairs = s->dim >> 1;
+    const unsigned grid = (unsigned)((npairs + kBlock - 1) / kBlock);
+    if (s->prec == QSV_C128)
+        hipLaunchKernelGGL(k_collapse<c128>, dim3(grid), dim3(kBlock), 0, s->stream, (c128 *)s->amps, npairs, ins, uint64_t(1) << q, outcome, scale);
+    else
+        hipLaunchKernelGGL(k_collapse<c64>, dim3(grid), dim3(kBlock), 0, s->stream, (c64 *)s->amps, npairs, ins, uint64_t(1) << q, outcome, scale);
+    s->launches++;
+    QSV_HIP(hipGetLastError());
+    return QSV_OK;
+}
+
+int marginals(qsv_state *s, const int *qubits, int m, double *out)
+{
+    const int tb = std::min(s->n, kTileBits);
+    uint64_t low_mask = 0, high_tile_mask = 0;
+    MargMap mm{};
+    mm.m = m;
+    for (int j = 0; j < m; j++) {
+        if (qubits[j] < tb) low_mask |= uint64_t(1) << qubits[j];
+        else high_tile_mask |= uint64_t(1) << (qubits[j] - tb);
+    }
+    InsertSet low_pos = insert_set_from_mask(low_mask);
+    mm.ml = low_pos.n;
+    for (int j = 0; j < m; j++) {
+        mm.low_rank[j] = -1;
+        mm.high_shift[j] = 0;
+        if (qubits[j] < tb) {
+            for (int r = 0; r < low_pos.n; r++)
+                if (low_pos.pos[r] == qubits[j]) mm.low_rank[j] = r;
+        } else {
+            mm.high_shift[j] = qubits[j] - tb;
+        }
+    }
+    const uint64_t ntiles = s->dim >> tb;
+    const size_t partial_bytes = sizeof(double) * (size_t)ntiles * ((size_t)1 << mm.ml);
+    const size_t out_bytes = sizeof(double) * ((size_t)1 << m);
+    int rc = ensure_aux(s, partial_bytes + out_bytes);
+    if (rc) return rc;
+    double *partial = (double *)s->aux;
+    double *dout = (double *)((char *)s->aux + partial_bytes);
+    if (s->prec == QSV_C128)
+        hipLaunchKernelGGL(k_marg_tile<c128>, dim3((unsigned)ntiles), dim3(kBlock), 0, s->stream, (const c128 *)s->amps, tb, low_mask, mm.ml, low_pos, partial);
+    else
+        hipLaunchKernelGGL(k_marg_tile<c64>, dim3((unsigned)ntiles), dim3(kBlock), 0, s->stream, (const c64 *)s->amps, tb, low_mask, mm.ml, low_pos, partial);
+    InsertSet high_ins = insert_set_from_mask(high_tile_mask);
+    const uint64_t each = ntiles >> high_ins.n;
+    hipLaunchKernelGGL(k_marg_final, dim3((unsigned)(uint64_t(1) << m)), dim3(kBlock), 0, s->stream, partial, mm, high_ins, each, dout);
+    s->launches += 2;
+    QSV_HIP(hipGetLastError());
+    QSV_HIP(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, s->stream));
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    return QSV_OK;
+}
+
+}  // namespace qsv

@@ -1,0 +1,281 @@
+"""Python binding of libqsv.so (the C ABI declared in include/qsv.h).
+
+The shared library is built in-tree by qpanda_amd/build.sh; loading fails
+loudly when it is missing -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import circuits as _circ
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqsv.so")
+
+C128, C64 = 0, 1
+
+EXPORTED = [
+    "qsv_create", "qsv_destroy", "qsv_reset_zero", "qsv_num_qubits", "qsv_precision",
+    "qsv_set_state", "qsv_get_state", "qsv_device_ptr", "qsv_stream", "qsv_sync",
+    "qsv_h", "qsv_x", "qsv_apply_1q", "qsv_diag", "qsv_rx", "qsv_ry", "qsv_rz",
+    "qsv_cphase", "qsv_cr", "qsv_iswap", "qsv_sqiswap", "qsv_cswap", "qsv_apply_2q",
+    "qsv_apply_block", "qsv_prob0", "qsv_norm2", "qsv_collapse", "qsv_measure",
+    "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
+    "qsv_sample_shots", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
+    "qsv_last_error", "qsv_device_count",
+]
+
+
+class QsvError(RuntimeError):
+    pass
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("exact", C.c_int), ("fuse_k", C.c_int), ("split", C.c_int),
+                ("scheduler", C.c_int), ("reserved", C.c_int * 4)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("gates", C.c_int64), ("launches", C.c_int64), ("blocks", C.c_int64),
+                ("tie_resolutions", C.c_int64), ("device_ms", C.c_double),
+                ("model_ms_pred", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QsvError(f"{LIB_PATH} is not built: run qpanda_amd/build.sh "
+                           "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, I, D, U64 = C.c_void_p, C.c_int, C.c_double, C.c_uint64
+        P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        sig = {
+            "qsv_create": ([I, I, C.POINTER(vp)], I),
+            "qsv_destroy": ([vp], I),
+            "qsv_reset_zero": ([vp], I),
+            "qsv_num_qubits": ([vp], I),
+            "qsv_precision": ([vp], I),
+            "qsv_set_state": ([vp, P], I),
+            "qsv_get_state": ([vp, P], I),
+            "qsv_device_ptr": ([vp, C.POINTER(vp)], I),
+            "qsv_stream": ([vp, C.POINTER(vp)], I),
+            "qsv_sync": ([vp], I),
+            "qsv_h": ([vp, I, U64, I], I),
+            "qsv_x": ([vp, I, U64], I),
+            "qsv_apply_1q": ([vp, I, P, U64], I),
+            "qsv_diag": ([vp, I, P, U64], I),
+            "qsv_rx": ([vp, I, D, I, U64], I),
+            "qsv_ry": ([vp, I, D, I, U64], I),
+            "qsv_rz": ([vp, I, D, I, U64], I),
+            "qsv_cphase": ([vp, I, I, D, D, U64], I),
+            "qsv_cr": ([vp, I, I, D, I], I),
+            "qsv_iswap": ([vp, I, I, I], I),
+            "qsv_sqiswap": ([vp, I, I, I, I], I),
+            "qsv_cswap": ([vp, I, I, I], I),
+            "qsv_apply_2q": ([vp, I, I, P], I),
+            "qsv_apply_block": ([vp, C.POINTER(I), I, P, I], I),
+            "qsv_prob0": ([vp, I, C.POINTER(D)], I),
+            "qsv_norm2": ([vp, C.POINTER(D)], I),
+            "qsv_collapse": ([vp, I, I, D], I),
+            "qsv_measure": ([vp, I, D, C.POINTER(I)], I),
+            "qsv_marginals": ([vp, C.POINTER(I), I, P], I),
+            "qsv_pmeasure": ([vp, C.POINTER(I), I, np.ctypeslib.ndpointer(dtype=np.uint64), P,
+                              C.POINTER(I)], I),
+            "qsv_expect_zz_chain": ([vp, C.POINTER(D)], I),
+            "qsv_run_circuit": ([vp, vp, C.c_int64, vp, C.POINTER(RunOpts), C.POINTER(C.c_int32),
+                                 vp, C.POINTER(RunStats)], I),
+            "qsv_sample_shots": ([vp, C.POINTER(I), I, C.c_int64, C.POINTER(C.c_int32), C.c_int64,
+                                  np.ctypeslib.ndpointer(dtype=np.uint64), C.POINTER(C.c_int64)], I),
+            "qsv_pm_next": ([C.c_int32], C.c_int32),
+            "qsv_pm_jump": ([C.c_int32, U64], C.c_int32),
+            "qsv_pm_seed_from_tm": ([I] * 6, C.c_int32),
+            "qsv_version": ([], C.c_char_p),
+            "qsv_last_error": ([], C.c_char_p),
+            "qsv_device_count": ([C.POINTER(I)], I),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise QsvError(f"qsv error {rc}: {lib().qsv_last_error().decode()}")
+
+
+def _ints(xs):
+    return (C.c_int * len(xs))(*[int(x) for x in xs])
+
+
+def _mat(u) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(u, np.complex128)).view(np.float64).ravel()
+
+
+def mask(qubits) -> int:
+    m = 0
+    for q in qubits:
+        m |= 1 << int(q)
+    return m
+
+
+class StateVector:
+    """A 2^n-amplitude state resident in HBM."""
+
+    def __init__(self, n: int, precision: int = C128):
+        h = C.c_void_p()
+        _check(lib().qsv_create(n, precision, C.byref(h)))
+        self.h = h
+        self.n = n
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            lib().qsv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state ----
+    def reset(self):
+        _check(lib().qsv_reset_zero(self.h))
+
+    def set(self, amps):
+        _check(lib().qsv_set_state(self.h, _mat(amps)))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(2 << self.n, np.float64)
+        _check(lib().qsv_get_state(self.h, out))
+        return out.view(np.complex128)
+
+    def sync(self):
+        _check(lib().qsv_sync(self.h))
+
+    def device_ptr(self) -> int:
+        p = C.c_void_p()
+        _check(lib().qsv_device_ptr(self.h, C.byref(p)))
+        return p.value
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        _check(lib().qsv_stream(self.h, C.byref(p)))
+        return p.value or 0
+
+    # ---- gates ----
+    def h_gate(self, t, ctrl=0, exact=False): _check(lib().qsv_h(self.h, t, ctrl, int(exact)))
+    def x(self, t, ctrl=0): _check(lib().qsv_x(self.h, t, ctrl))
+    def u1(self, t, u, ctrl=0): _check(lib().qsv_apply_1q(self.h, t, _mat(u), ctrl))
+    def diag(self, t, d0, d1, ctrl=0):
+        _check(lib().qsv_diag(self.h, t, np.array([d0.real, d0.imag, d1.real, d1.imag]), ctrl))
+    def rx(self, t, th, dagger=0, ctrl=0): _check(lib().qsv_rx(self.h, t, th, dagger, ctrl))
+    def ry(self, t, th, dagger=0, ctrl=0): _check(lib().qsv_ry(self.h, t, th, dagger, ctrl))
+    def rz(self, t, th, dagger=0, ctrl=0): _check(lib().qsv_rz(self.h, t, th, dagger, ctrl))
+    def cphase(self, a, b, z, ctrl=0): _check(lib().qsv_cphase(self.h, a, b, z.real, z.imag, ctrl))
+    def cr(self, a, b, th, dagger=0): _check(lib().qsv_cr(self.h, a, b, th, dagger))
+    def iswap(self, a, b, dagger=0): _check(lib().qsv_iswap(self.h, a, b, dagger))
+    def sqiswap(self, a, b, dagger=0, exact=False): _check(lib().qsv_sqiswap(self.h, a, b, dagger, int(exact)))
+    def cswap(self, c, a, b): _check(lib().qsv_cswap(self.h, c, a, b))
+    def u2(self, qa, qb, u): _check(lib().qsv_apply_2q(self.h, qa, qb, _mat(u)))
+
+    def block(self, qubits, u, split=True):
+        _check(lib().qsv_apply_block(self.h, _ints(qubits), len(qubits), _mat(u), int(split)))
+
+    # ---- readout ----
+    def prob0(self, q) -> float:
+        v = C.c_double()
+        _check(lib().qsv_prob0(self.h, q, C.byref(v)))
+        return v.value
+
+    def norm2(self) -> float:
+        v = C.c_double()
+        _check(lib().qsv_norm2(self.h, C.byref(v)))
+        return v.value
+
+    def expect_zz_chain(self) -> float:
+        v = C.c_double()
+        _check(lib().qsv_expect_zz_chain(self.h, C.byref(v)))
+        return v.value
+
+    def collapse(self, q, outcome, scale):
+        _check(lib().qsv_collapse(self.h, q, outcome, scale))
+
+    def measure(self, q, r) -> int:
+        o = C.c_int()
+        _check(lib().qsv_measure(self.h, q, r, C.byref(o)))
+        return o.value
+
+    def marginals(self, qubits) -> np.ndarray:
+        out = np.empty(1 << len(qubits), np.float64)
+        _check(lib().qsv_marginals(self.h, _ints(qubits), len(qubits), out))
+        return out
+
+    def pmeasure(self, qubits):
+        idx = np.zeros(10, np.uint64)
+        prob = np.zeros(10, np.float64)
+        cnt = C.c_int()
+        _check(lib().qsv_pmeasure(self.h, _ints(qubits), len(qubits), idx, prob, C.byref(cnt)))
+        return list(zip(idx[:cnt.value].tolist(), prob[:cnt.value].tolist()))
+
+    def sample_shots(self, qubits, gate_draws: int, rng_state: int, shots: int):
+        """Reference shot loop from the current state (which is left unchanged):
+        each shot skips `gate_draws` draws, then measures `qubits` in order.
+        Returns (words, rng_state, tie_resolutions)."""
+        out = np.zeros(shots, np.uint64)
+        rs = C.c_int32(rng_state)
+        ties = C.c_int64(0)
+        _check(lib().qsv_sample_shots(self.h, _ints(qubits), len(qubits), gate_draws, C.byref(rs),
+                                      shots, out, C.byref(ties)))
+        return out, rs.value, ties.value
+
+    def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=True, scheduler=0,
+            rng_state=None, outcomes=None) -> RunStats:
+        g = np.ascontiguousarray(circ.gates)
+        m = np.ascontiguousarray(circ.mats, np.complex128)
+        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler))
+        st = RunStats()
+        rs = C.c_int32(rng_state if rng_state is not None else 1)
+        self._last_rng = None
+        _check(lib().qsv_run_circuit(self.h, g.ctypes.data_as(C.c_void_p), g.shape[0],
+                                     m.ctypes.data_as(C.c_void_p) if m.size else None,
+                                     C.byref(opts), C.byref(rs) if rng_state is not None else None,
+                                     outcomes.ctypes.data_as(C.c_void_p) if outcomes is not None else None,
+                                     C.byref(st)))
+        self._last_rng = rs.value
+        return st
+
+    @property
+    def last_rng(self):
+        return self._last_rng
+
+
+def device_count() -> int:
+    c = C.c_int()
+    _check(lib().qsv_device_count(C.byref(c)))
+    return c.value
+
+
+def pm_next(s):
+    return lib().qsv_pm_next(s)
+
+
+def pm_jump(s, k):
+    return lib().qsv_pm_jump(s, k)
