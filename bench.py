@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""Benchmark: gate-applications/s and achieved HBM bandwidth at 30 qubits.
+
+Workload: a 30-qubit random circuit of depth 200 (per layer: H/RX/RY/RZ on every
+qubit with angles uniform in [0, 2*pi); a seeded perfect pairing with CNOT or CZ,
+10% of the pairs replaced by Haar-random two-qubit unitaries), 9,000 logical
+gates, synthetic, generated from a fixed seed.
+
+Paths:
+  plain  per-gate complex128 sweeps (one launch per gate), the FP64 reference path;
+  fused  complex64 state, gates fused into <= k-qubit blocks run as FP32 MFMA
+         complex GEMMs with the hi+lo split matrix product (when built).
+A step is one full circuit execution from |0...0> on the headline path (fused
+when available, else plain).  The plain path additionally runs once as the
+FP64 comparison; fidelity |<plain|fused>|^2 is computed on the device in FP64.
+
+Roofline: the dominant kernel class of the headline path, timed per launch with
+HIP events on the state's stream inside the timed steps; achieved = algorithmic
+bytes (every amplitude a gate may change, read and written once) / time.
+
+CPU baseline: QPanda's own CPU simulator (oracle/_ref, OpenMP build) on a bounded
+strided sample of the same circuit at the same size, on this host's cores.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=None)
+    p.add_argument("--warmup", type=int, default=None)
+    p.add_argument("--n", type=int, default=30)
+    p.add_argument("--depth", type=int, default=200)
+    p.add_argument("--haar", type=float, default=0.1)
+    p.add_argument("--seed", type=int, default=2025)
+    p.add_argument("--fuse-k", type=int, default=4)
+    p.add_argument("--scheduler", choices=["greedy", "learned"], default="greedy")
+    p.add_argument("--path", choices=["auto", "plain", "fused"], default="auto")
+    p.add_argument("--no-plain", action="store_true", help="skip the FP64 comparison run")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=20.0)
+    p.add_argument("--cpu-stride", type=int, default=97)
+    p.add_argument("--probe", action="store_true",
+                   help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
+    return p.parse_args()
+
+
+def probe(a):
+    """Short fixed launch sequence for hardware-counter runs: the copy kernel
+    moves exactly 2 * 16 * 2^n bytes, which scales FETCH_SIZE / WRITE_SIZE;
+    then RX sweeps on qubits 0, n/2 and n-1 (same algorithmic bytes)."""
+    from qpanda_amd import circuits as C
+    from qpanda_amd import engine as E
+    out = {"n": a.n, "copy_bytes": 2 * 16 * (1 << a.n)}
+    with E.StateVector(a.n, E.C128) as s1, E.StateVector(a.n, E.C128) as s2:
+        s2.copy_from(s1)
+        b = C._Builder(a.n)
+        for q in (0, a.n // 2, a.n - 1):
+            b.add(C.RX, q, theta=0.5)
+        st = s1.run(b.build(), profile=True)
+        out["rx_ms"] = st.class_ms[0] / max(1, st.class_launches[0])
+        out["rx_algorithmic_bytes"] = 2 * 16 * (1 << a.n)
+    print(json.dumps(out), flush=True)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(circ, seconds, stride):
+    """QPanda's CPU simulator (reference build) on records 0, stride, 2*stride, ...
+    of the benchmark circuit; falls back to the oracle port when the reference
+    build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "bench.qsvc")
+        circ.save(path)
+        if O.ref_available():
+            r = O.ref_time(path, stride, seconds, omp=True, threads=threads)
+            return {"value": r["gates"] / r["seconds"], "unit": "gate-apps/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": (f"QPanda CPUQuantumGates (OpenMP build, OMP_NUM_THREADS={threads}) at "
+                               f"n={circ.n}, c128: records 0,{stride},{2 * stride},... of the benchmark "
+                               f"circuit, {r['gates']} gates ({r['reference_calls']} reference calls; CZ as "
+                               f"CR(pi), Haar pairs as an 18-call KAK-shaped template) in "
+                               f"{r['seconds']:.1f} s, state allocation excluded")}
+        # port: the oracle restatement, single thread
+        from qpanda_amd.circuits import Circuit
+        st = O.State(circ.n)
+        t0 = time.time()
+        done = 0
+        for i in range(0, circ.ngates, stride):
+            st.run(Circuit(circ.n, circ.gates[i:i + 1].copy(), circ.mats))
+            done += 1
+            if time.time() - t0 > seconds:
+                break
+        dt = time.time() - t0
+        del st
+        return {"value": done / dt, "unit": "gate-apps/s", "cores": 1, "kind": "port",
+                "sample": f"oracle restatement, 1 thread, n={circ.n}, {done} strided gates in {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU design is "
+              "described, not run", file=sys.stderr)
+        sys.exit(2)
+    if a.probe:
+        probe(a)
+        return
+    from qpanda_amd import circuits as C
+    from qpanda_amd import engine as E
+
+    circ = C.random_layers(a.n, a.depth, a.seed, haar_frac=a.haar)
+    G = circ.ngates
+    log(f"workload n={a.n} depth={a.depth} gates={G} counts={circ.counts()}")
+
+    fused_ok = a.path in ("auto", "fused")
+    if fused_ok:
+        try:
+            with E.StateVector(2, E.C64) as probe:
+                probe.run(C.ghz(2), fuse_k=2)
+        except E.QsvError as e:
+            if a.path == "fused":
+                raise
+            log("fused path unavailable:", e)
+            fused_ok = False
+    headline = "fused" if fused_ok else "plain"
+    steps = a.steps if a.steps is not None else (3 if headline == "fused" else 1)
+    warmup = a.warmup if a.warmup is not None else (1 if headline == "fused" else 0)
+    sched = 1 if a.scheduler == "learned" else 0
+
+    plain = None
+    result = {}
+    sv_plain = None
+    if headline == "plain" or not a.no_plain:
+        sv_plain = E.StateVector(a.n, E.C128)
+        log("plain path: one run (FP64 comparison)" if headline == "fused" else "plain path")
+
+    def run_plain(profile):
+        sv_plain.reset()
+        return sv_plain.run(circ, profile=profile)
+
+    if headline == "plain":
+        for _ in range(warmup):
+            run_plain(False)
+        sv_plain.sync()
+        t0 = time.perf_counter()
+        stats = []
+        for _ in range(steps):
+            stats.append(run_plain(True))
+        sv_plain.sync()
+        wall = time.perf_counter() - t0
+        head_stats = stats
+    else:
+        if sv_plain is not None:
+            plain = run_plain(True)
+            sv_plain.sync()
+        sv = E.StateVector(a.n, E.C64)
+        k = a.fuse_k
+        for _ in range(warmup):
+            sv.reset()
+            sv.run(circ, fuse_k=k, split=True, scheduler=sched)
+        sv.sync()
+        t0 = time.perf_counter()
+        stats = []
+        for _ in range(steps):
+            sv.reset()
+            stats.append(sv.run(circ, fuse_k=k, split=True, scheduler=sched, profile=True))
+        sv.sync()
+        wall = time.perf_counter() - t0
+        head_stats = stats
+        if sv_plain is not None:
+            result["fidelity_vs_fp64"] = sv_plain.fidelity(sv)
+        result["fused"] = {"k": k, "split": True, "scheduler": a.scheduler,
+                           "blocks": int(stats[-1].blocks),
+                           "device_ms": stats[-1].device_ms,
+                           "predicted_ms": stats[-1].model_ms_pred}
+        sv.close()
+
+    ms_per_step = wall * 1e3 / steps
+    value = G / (wall / steps)
+    # dominant kernel class over the timed steps
+    agg = {}
+    for st in head_stats:
+        for name, c in st.classes().items():
+            d = agg.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            for key in d:
+                d[key] += c[key]
+    dom = max(agg, key=lambda nm: agg[nm]["ms"])
+    d = agg[dom]
+    total_bytes = sum(v["bytes"] for v in agg.values())
+    total_ms = sum(v["ms"] for v in agg.values())
+    per_launch_ms = d["ms"] / d["launches"]
+    if dom == "fused":
+        achieved = d["flops"] / (d["ms"] * 1e9)
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        achieved = d["bytes"] / (d["ms"] * 1e6)
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel_class"] = dom
+    roof["avg_launch_ms"] = per_launch_ms
+    roof["algorithmic_bytes_per_launch"] = d["bytes"] / d["launches"]
+    result["hbm_gbs"] = total_bytes / (total_ms * 1e6)
+    result["hbm_frac"] = result["hbm_gbs"] / HBM_PEAK_GBS
+    result["kernel_classes"] = {k2: {"launches": int(v["launches"] / steps), "ms": v["ms"] / steps,
+                                     "GBs": v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0.0}
+                                for k2, v in agg.items()}
+    if plain is not None:
+        result["plain"] = {"gate_apps_per_s": G / (plain.device_ms / 1e3), "device_ms": plain.device_ms,
+                           "hbm_gbs": sum(plain.class_bytes) / (sum(plain.class_ms) * 1e6)}
+        result["speedup_vs_plain"] = plain.device_ms / head_stats[-1].device_ms
+
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(circ, a.cpu_seconds, a.cpu_stride)
+    if sv_plain is not None:
+        sv_plain.close()
+
+    line = {
+        "metric": "gate-apps/s at 30q (random circuit, depth 200)",
+        "value": value,
+        "unit": "gate-apps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 mfma, fp64 reductions)" if headline == "fused" else "c128 (fp64)",
+        "data": "synthetic seeded circuit",
+        "config": {"workload": f"random circuit n={a.n} depth={a.depth} haar_frac={a.haar} "
+                               f"seed={a.seed} ({G} gates)",
+                   "path": headline, "n_qubits": a.n, "depth": a.depth, "gates": G},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -118,6 +118,15 @@ int qsv_precision(const qsv_state *s);
 int qsv_set_state(qsv_state *s, const double *amps);
 int qsv_get_state(qsv_state *s, double *amps);
 
+/* Device-side copy of src into dst (same qubit count; precisions may differ,
+ * converting c128 <-> c64).  A streaming kernel, used for snapshots and for
+ * comparing paths.  No reference counterpart. */
+int qsv_copy_state(qsv_state *dst, qsv_state *src);
+
+/* <a|b> in FP64 (either precision; same qubit count).  Fidelity of two states is
+ * |<a|b>|^2 / (<a|a><b|b>).  No reference counterpart. */
+int qsv_inner(qsv_state *a, qsv_state *b, double *re, double *im);
+
 /* Raw device pointer of the amplitude array (interleaved complex of the state's
  * precision) and the handle's HIP stream, for callers that keep data in HBM. */
 int qsv_device_ptr(qsv_state *s, void **dptr);
@@ -236,8 +245,22 @@ typedef struct qsv_run_opts {
     int fuse_k;         /* 0: per-gate path; 2..5: fused blocks of <= k qubits  */
     int split;          /* fused blocks: 1 = hi+lo split matrix product         */
     int scheduler;      /* 0: greedy fusion, 1: learned cost model              */
-    int reserved[4];
+    int profile;        /* 1: time every launch with events (per-class stats)   */
+    int reserved[3];
 } qsv_run_opts;
+
+/* Kernel classes for per-launch profiling. */
+enum {
+    QSV_K_DENSE1 = 0,   /* 2x2 pair sweep (H, RX, RY, U1, controlled forms)      */
+    QSV_K_DIAG = 1,     /* diagonal pair sweep (RZ)                              */
+    QSV_K_PERM = 2,     /* permutations (X, CNOT, Toffoli, Fredkin) and iSWAPs    */
+    QSV_K_PHASE = 3,    /* one-subspace phase (CR, CZ)                           */
+    QSV_K_BLOCK64 = 4,  /* dense block in FP64 (two-qubit unitaries, c128 blocks) */
+    QSV_K_FUSED = 5,    /* fused FP32 MFMA block (c64)                           */
+    QSV_K_READOUT = 6,  /* reductions, collapse                                  */
+    QSV_K_OTHER = 7,
+    QSV_K_CLASSES = 8
+};
 
 typedef struct qsv_run_stats {
     int64_t gates;          /* logical gates applied                     */
@@ -246,7 +269,25 @@ typedef struct qsv_run_stats {
     int64_t tie_resolutions;/* measurements decided in serial order      */
     double  device_ms;      /* device time between events around the run */
     double  model_ms_pred;  /* scheduler's predicted time (fused path)    */
+    /* filled when opts.profile != 0: per kernel class, launches, device time
+     * (event to event on the state's stream), algorithmic HBM bytes (every
+     * amplitude the gate may change read once and written once) and flops */
+    int64_t class_launches[QSV_K_CLASSES];
+    double  class_ms[QSV_K_CLASSES];
+    double  class_bytes[QSV_K_CLASSES];
+    double  class_flops[QSV_K_CLASSES];
 } qsv_run_stats;
+
+/* Greedy fusion plan of a gate list (host only, no device needed): blocks of at
+ * most max_k qubits in emission order.  For block b: k_out[b] qubits (0 for a
+ * MEASURE/RESET record that runs alone), qubits_out[5b..5b+4] (row bit j =
+ * bit(qubits_out[5b+j]), -1 padded), mats_out[2048b..] the 2^k x 2^k complex
+ * matrix (row-major re, im), ngates_out[b] the logical gates it holds (the
+ * record index for k = 0).  Any output pointer may be NULL.  No reference
+ * counterpart (the reference applies one gate per sweep). */
+int qsv_fuse_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
+                  int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                  int64_t *ngates_out, int64_t *nblocks);
 
 int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngates,
                     const double *mats, const qsv_run_opts *opts,

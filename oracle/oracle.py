@@ -217,11 +217,14 @@ def ref_run(circ_path: str, out_prefix: str, seed: int, shots: int, omp: bool = 
     subprocess.check_call([exe, "run", circ_path, out_prefix, str(seed), str(shots)])
 
 
-def ref_time(circ_path: str, gates: int, omp: bool = True, threads: int | None = None) -> dict:
+def ref_time(circ_path: str, stride: int, max_seconds: float, omp: bool = True,
+             threads: int | None = None) -> dict:
+    """Time QPanda's own CPU simulator on records 0, stride, 2*stride, ... of a
+    circuit file until max_seconds has passed (allocation excluded)."""
     import json
     exe = os.path.join(REF_DIR, "qpanda_ref_omp" if omp else "qpanda_ref")
     env = dict(os.environ)
     if threads is not None:
         env["OMP_NUM_THREADS"] = str(threads)
-    out = subprocess.check_output([exe, "time", circ_path, str(gates)], env=env)
+    out = subprocess.check_output([exe, "time", circ_path, str(stride), str(max_seconds)], env=env)
     return json.loads(out.decode().strip().splitlines()[-1])

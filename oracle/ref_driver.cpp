@@ -19,8 +19,9 @@
 //   run  CIRCUIT OUTPREFIX SEED SHOTS   -> OUTPREFIX.state  (2^n x {f64 re, f64 im},
 //                                          state before the first MEASURE of shot 0)
 //                                          OUTPREFIX.shots  (SHOTS x u64, bit c = creg c)
-//   time CIRCUIT GATES                  -> one JSON line: seconds for the first GATES
-//                                          records (CZ as CR(pi); U2 as a 3-CNOT template)
+//   time CIRCUIT STRIDE MAX_SECONDS     -> one JSON line: wall time of records 0, STRIDE,
+//                                          2*STRIDE, ... until MAX_SECONDS has passed
+//                                          (CZ as CR(pi); U2 as a 3-CNOT template)
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -151,10 +152,12 @@ static int cmd_run(int argc, char **argv)
 
 static int cmd_time(int argc, char **argv)
 {
-    if (argc < 4) return 2;
+    if (argc < 5) return 2;
     Circuit c;
     if (!load_circuit(argv[2], c)) { std::fprintf(stderr, "cannot read %s\n", argv[2]); return 1; }
-    long limit = std::atol(argv[3]);
+    long stride = std::atol(argv[3]);
+    double max_s = std::atof(argv[4]);
+    if (stride < 1) stride = 1;
     std::memset(&g_tm, 0, sizeof g_tm);
     g_tm.tm_year = 1 - 70;
 
@@ -166,20 +169,21 @@ static int cmd_time(int argc, char **argv)
     auto t1 = std::chrono::steady_clock::now();
     std::map<size_t, int> creg;
     long done = 0, sweeps = 0;
-    for (const qsv_gate &r : c.gates) {
-        if (done >= limit) break;
+    for (size_t i = 0; i < c.gates.size(); i += (size_t)stride) {
+        const qsv_gate &r = c.gates[i];
         if (r.kind == QSV_GATE_U2) { apply_u2_template(g, r.q0, r.q1); sweeps += 18; }
         else if (r.kind == QSV_GATE_U1) { g.RZ(r.q0, 0.1, 0, 0); g.RY(r.q0, 0.2, 0, 0); g.RZ(r.q0, 0.3, 0, 0); sweeps += 3; }
         else { if (!apply(g, r, creg)) return 1; sweeps += 1; }
         done++;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count() > max_s) break;
     }
     auto t2 = std::chrono::steady_clock::now();
     double init_s = std::chrono::duration<double>(t1 - t0).count();
     double run_s = std::chrono::duration<double>(t2 - t1).count();
     const char *omp = std::getenv("OMP_NUM_THREADS");
-    std::printf("{\"n\": %d, \"gates\": %ld, \"reference_calls\": %ld, \"seconds\": %.6f, "
+    std::printf("{\"n\": %d, \"gates\": %ld, \"stride\": %ld, \"reference_calls\": %ld, \"seconds\": %.6f, "
                 "\"init_seconds\": %.6f, \"omp_num_threads\": \"%s\"}\n",
-                c.n, done, sweeps, run_s, init_s, omp ? omp : "");
+                c.n, done, stride, sweeps, run_s, init_s, omp ? omp : "");
     return 0;
 }
 
@@ -187,6 +191,6 @@ int main(int argc, char **argv)
 {
     if (argc >= 2 && std::strcmp(argv[1], "run") == 0) return cmd_run(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "time") == 0) return cmd_time(argc, argv);
-    std::fprintf(stderr, "usage: %s run CIRCUIT OUTPREFIX SEED SHOTS | time CIRCUIT GATES\n", argv[0]);
+    std::fprintf(stderr, "usage: %s run CIRCUIT OUTPREFIX SEED SHOTS | time CIRCUIT STRIDE MAX_SECONDS\n", argv[0]);
     return 2;
 }

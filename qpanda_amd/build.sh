@@ -8,15 +8,25 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclan
        -Wall -Wno-unused-function -I"$HERE/../include")
 mkdir -p _build
 OBJS=()
+PIDS=()
 for src in csrc/*.hip csrc/*.cpp; do
   obj="_build/$(basename "$src").o"
-  if [[ "$src" == *.hip ]]; then
-    "$HIPCC" "${FLAGS[@]}" -c "$src" -o "$obj"
-  else
-    "$HIPCC" "${FLAGS[@]}" -x hip -c "$src" -o "$obj"
-  fi
   OBJS+=("$obj")
+  if [[ "$src" == *.hip ]]; then
+    "$HIPCC" "${FLAGS[@]}" -c "$src" -o "$obj" 2> "$obj.log" &
+  else
+    "$HIPCC" "${FLAGS[@]}" -x hip -c "$src" -o "$obj" 2> "$obj.log" &
+  fi
+  PIDS+=($!)
 done
+fail=0
+for i in "${!PIDS[@]}"; do
+  if ! wait "${PIDS[$i]}"; then
+    fail=1
+  fi
+  grep -v -e "packed-fp32-ops" -e "^$" "${OBJS[$i]}.log" >&2 || true
+done
+[[ $fail == 0 ]] || { echo "build failed" >&2; exit 1; }
 "$HIPCC" -shared -fPIC --offload-arch=gfx950 -o libqsv.so.tmp "${OBJS[@]}"
 mv libqsv.so.tmp libqsv.so
 echo "built $HERE/libqsv.so"

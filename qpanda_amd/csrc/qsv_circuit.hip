@@ -140,6 +140,83 @@ static int apply_record(qsv_state *s, const qsv_gate &r, const double *mats_host
     }
 }
 
+// Kernel class and algorithmic cost of one record on the per-gate path:
+// bytes = 2 * S * (amplitudes the gate may change), flops = arithmetic on them.
+int record_class(const qsv_state *s, const qsv_gate &r, double *bytes, double *flops)
+{
+    const double S = (double)amp_bytes(s), N = (double)s->dim;
+    double touched = N, fl = 0;
+    int cls = QSV_K_OTHER;
+    switch (r.kind) {
+    case QSV_GATE_H: case QSV_GATE_RX: case QSV_GATE_RY: case QSV_GATE_U1:
+        cls = QSV_K_DENSE1; fl = 14 * N; break;
+    case QSV_GATE_RZ: cls = QSV_K_DIAG; fl = 6 * N; break;
+    case QSV_GATE_X: cls = QSV_K_PERM; break;
+    case QSV_GATE_CNOT: cls = QSV_K_PERM; touched = r.q0 == r.q1 ? 0 : N / 2; break;
+    case QSV_GATE_TOFFOLI: case QSV_GATE_CSWAP: cls = QSV_K_PERM; touched = N / 4; break;
+    case QSV_GATE_ISWAP: cls = QSV_K_PERM; touched = N / 2; fl = 0; break;
+    case QSV_GATE_SQISWAP: cls = QSV_K_DENSE1; touched = N / 2; fl = 14 * N / 2; break;
+    case QSV_GATE_CZ: case QSV_GATE_CR: cls = QSV_K_PHASE; touched = N / 4; fl = 6 * N / 4; break;
+    case QSV_GATE_U2: cls = QSV_K_BLOCK64; fl = 32 * N; break;
+    case QSV_GATE_MEASURE: case QSV_GATE_RESET: cls = QSV_K_READOUT; touched = N; break;
+    default: break;
+    }
+    *bytes = 2 * S * touched;
+    *flops = fl;
+    return cls;
+}
+
+Profiler::Profiler(qsv_state *s, bool on) : s_(s), on_(on) {}
+
+Profiler::~Profiler()
+{
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+}
+
+int Profiler::begin()
+{
+    if (!on_) return QSV_OK;
+    if (events_.size() == cls_.size()) {       // need an opening event
+        hipEvent_t e;
+        QSV_HIP(hipEventCreate(&e));
+        QSV_HIP(hipEventRecord(e, s_->stream));
+        events_.push_back(e);
+        cls_.push_back(-1);
+        bytes_.push_back(0);
+        flops_.push_back(0);
+    }
+    return QSV_OK;
+}
+
+int Profiler::end(int cls, double bytes, double flops)
+{
+    if (!on_) return QSV_OK;
+    hipEvent_t e;
+    QSV_HIP(hipEventCreate(&e));
+    QSV_HIP(hipEventRecord(e, s_->stream));
+    events_.push_back(e);
+    cls_.push_back(cls);
+    bytes_.push_back(bytes);
+    flops_.push_back(flops);
+    return QSV_OK;
+}
+
+int Profiler::collect(qsv_run_stats *st)
+{
+    if (!on_ || events_.empty()) return QSV_OK;
+    QSV_HIP(hipEventSynchronize(events_.back()));
+    for (size_t i = 1; i < events_.size(); i++) {
+        if (cls_[i] < 0) continue;
+        float ms = 0;
+        QSV_HIP(hipEventElapsedTime(&ms, events_[i - 1], events_[i]));
+        st->class_launches[cls_[i]] += 1;
+        st->class_ms[cls_[i]] += ms;
+        st->class_bytes[cls_[i]] += bytes_[i];
+        st->class_flops[cls_[i]] += flops_[i];
+    }
+    return QSV_OK;
+}
+
 }  // namespace qsv
 
 using namespace qsv;
@@ -163,14 +240,22 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
     QSV_HIP(hipEventCreate(&e0));
     QSV_HIP(hipEventCreate(&e1));
     const int64_t launches0 = s->launches, ties0 = s->tie_resolutions;
+    Profiler prof(s, o.profile != 0);
     QSV_HIP(hipEventRecord(e0, s->stream));
     int64_t blocks = 0;
     double pred_ms = 0;
     if (o.fuse_k >= 2) {
-        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms);
+        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof);
     } else {
-        for (int64_t i = 0; i < ngates && !rc; i++)
-            rc = apply_record(s, gates[i], mats, o.exact, rng, outcomes);
+        for (int64_t i = 0; i < ngates && !rc; i++) {
+            rc = prof.begin();
+            if (!rc) rc = apply_record(s, gates[i], mats, o.exact, rng, outcomes);
+            if (!rc) {
+                double bytes, flops;
+                const int cls = record_class(s, gates[i], &bytes, &flops);
+                rc = prof.end(cls, bytes, flops);
+            }
+        }
     }
     hipError_t er = hipEventRecord(e1, s->stream);
     if (!rc && er != hipSuccess) rc = hip_fail(er, "hipEventRecord");
@@ -183,12 +268,14 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
         stats->gates = ngates;
         stats->launches = s->launches - launches0;
         stats->blocks = blocks;
         stats->tie_resolutions = s->tie_resolutions - ties0;
         stats->device_ms = ms;
         stats->model_ms_pred = pred_ms;
+        if (!rc) rc = prof.collect(stats);
     }
     return rc;
 }

@@ -177,6 +177,24 @@ int qsv_get_state(qsv_state *s, double *amps)
     return QSV_OK;
 }
 
+int qsv_copy_state(qsv_state *dst, qsv_state *src)
+{
+    CHECK_STATE(dst);
+    CHECK_STATE(src);
+    if (dst->n != src->n) return fail(QSV_ERR_ARG, "qubit counts differ");
+    if (dst == src) return QSV_OK;
+    return copy_state(dst, src);
+}
+
+int qsv_inner(qsv_state *a, qsv_state *b, double *re, double *im)
+{
+    CHECK_STATE(a);
+    CHECK_STATE(b);
+    if (!re || !im) return fail(QSV_ERR_ARG, "null");
+    if (a->n != b->n) return fail(QSV_ERR_ARG, "qubit counts differ");
+    return inner(a, b, re, im);
+}
+
 int qsv_device_ptr(qsv_state *s, void **dptr)
 {
     CHECK_STATE(s);

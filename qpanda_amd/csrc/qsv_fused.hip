@@ -1,11 +1,250 @@
-// qsv_fused.hip -- placeholder: dense blocks on C64 states use the FP64 block kernel.
+// qsv_fused.hip -- fused k-qubit blocks (k = 4, 5) as FP32 complex GEMMs on MFMA.
+//
+// A block applies a dense 2^k x 2^k complex matrix U to every group of 2^k
+// amplitudes that differ only in the block's qubits Q.  Written as a real GEMM,
+// Y = M X with M the 2d x 2d real form of U (d = 2^k):
+//     M[2r][2c] = Re U[r][c],  M[2r][2c+1] = -Im U[r][c],
+//     M[2r+1][2c] = Im U[r][c], M[2r+1][2c+1] = Re U[r][c],
+// and X's column j holding group j's amplitudes as (re, im) row pairs.
+//
+// Tiling (one workgroup of 4 waves per tile):
+//   L  = Q u {the 7 lowest qubits not in Q}; L always contains qubits 0..6, so a
+//        tile -- the 2^(k+7) amplitudes whose non-L bits are fixed -- consists of
+//        contiguous runs of >= 128 amplitudes (1 KiB).
+//   load:  16-byte loads of amplitude pairs into LDS, stored as separate re / im
+//          planes indexed by the compact position e over L (conflict-free reads);
+//   math:  wave w owns the 32 groups (columns) w*32..w*32+31 of the tile; per
+//          K-step s it feeds amplitude s of its 32 columns (re in lanes 0-31,
+//          im in lanes 32-63) as the B operand of v_mfma_f32_32x32x2f32, with M's
+//          columns 2s, 2s+1 as A (held in registers for the whole tile);
+//          split = 1 adds a second product with M_lo = fp32(M - fp32(M)), both
+//          accumulated in FP32;
+//   store: results go back to the planes, then 16-byte coalesced stores.
+// Per tile the kernel moves 2 * 8 * 2^(k+7) bytes and performs
+// 2 * (2d)^2 * 128 flops per product, so k = 4 sits near the FP32 MFMA ridge and
+// k = 5 (or k = 4 with the split) is MFMA-bound.
+//
+// MFMA 32x32x2 f32 operand layout (one block): A[i][k] in lane i + 32k,
+// B[k][j] in lane j + 32k, D[i][j] in lane j + 32*((i/4)%2), register
+// (i/8)*4 + i%4.
 #include "qsv_internal.h"
+#include "qsv_sched.h"
 
 namespace qsv {
 
-int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int)
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+struct FusedArgs {
+    uint64_t off_high[32];   // address offset of the tile bits above the 7-bit run
+    InsertSet tile_ins;      // L positions, to expand a tile index into its base
+    int erow[32];            // compact tile position of row (amplitude) r
+    int col_rank[7];         // rank in L of the i-th of the 7 free tile bits
+};
+
+template <int KQ, bool SPLIT>
+__global__ __launch_bounds__(256) void k_fused_mfma(float2 *__restrict__ a, uint64_t ntiles, FusedArgs fa,
+                                                     const float *__restrict__ mops)
 {
-    return launch_block(s, qubits, k, u, 0);
+    constexpr int D = 1 << KQ;               // amplitudes per group
+    constexpr int TB = KQ + 7;               // tile bits
+    constexpr int TSIZE = 1 << TB;           // amplitudes per tile
+    constexpr int HALVES = (2 * D) / 32;     // 32-row output halves: 1 (k=4) or 2 (k=5)
+    constexpr int KSTEPS = D;                // K = 2D real rows, 2 per step
+    __shared__ float planes[2][TSIZE];     // [0] = re, [1] = im
+    float *const pre = planes[0];
+    float *const pim = planes[1];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const uint64_t base = expand_index(tile, fa.tile_ins);
+
+    // A operands (the matrix), operand-ready layout: mops[(h*KSTEPS + s)*64 + lane]
+    float ahi[HALVES][KSTEPS];
+    float alo[SPLIT ? HALVES : 1][SPLIT ? KSTEPS : 1];
+#pragma unroll
+    for (int h = 0; h < HALVES; h++)
+#pragma unroll
+        for (int s = 0; s < KSTEPS; s++) {
+            ahi[h][s] = mops[(h * KSTEPS + s) * 64 + lane];
+            if (SPLIT) alo[SPLIT ? h : 0][SPLIT ? s : 0] = mops[(HALVES * KSTEPS + h * KSTEPS + s) * 64 + lane];
+        }
+
+    // load the tile: element pairs (e, e+1) are adjacent in memory (bit 0 is in L)
+#pragma unroll
+    for (int i = 0; i < TSIZE / 512; i++) {
+        const int e = 2 * (tid + 256 * i);
+        const uint64_t addr = base + (uint64_t)(e & 127) + fa.off_high[e >> 7];
+        const float4 v = *reinterpret_cast<const float4 *>(a + addr);
+        pre[e] = v.x; pim[e] = v.y;
+        pre[e + 1] = v.z; pim[e + 1] = v.w;
+    }
+    __syncthreads();
+
+    // this lane's column within the wave's 32 and its compact position bits
+    const int col = wave * 32 + (lane & 31);
+    int ecol = 0;
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+        if (col >> i & 1) ecol |= 1 << fa.col_rank[i];
+    const float *const bplane = planes[lane >> 5];   // lanes 32-63 feed the im rows
+
+    floatx16 acc[HALVES];
+#pragma unroll
+    for (int h = 0; h < HALVES; h++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[h][v] = 0.0f;
+
+#pragma unroll
+    for (int s = 0; s < KSTEPS; s++) {
+        const int e = fa.erow[s] | ecol;
+        const float b = bplane[e];
+#pragma unroll
+        for (int h = 0; h < HALVES; h++) {
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], b, acc[h], 0, 0, 0);
+            if (SPLIT) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(alo[SPLIT ? h : 0][SPLIT ? s : 0], b, acc[h], 0, 0, 0);
+        }
+    }
+    // all B reads of this wave precede its writes (data dependence); other
+    // waves touch other columns, so no barrier is needed before the write-back
+#pragma unroll
+    for (int h = 0; h < HALVES; h++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int m = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);   // real output row
+            const int e = fa.erow[m >> 1] | ecol;
+            if (m & 1) pim[e] = acc[h][v];
+            else pre[e] = acc[h][v];
+        }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TSIZE / 512; i++) {
+        const int e = 2 * (tid + 256 * i);
+        const uint64_t addr = base + (uint64_t)(e & 127) + fa.off_high[e >> 7];
+        float4 v;
+        v.x = pre[e]; v.y = pim[e];
+        v.z = pre[e + 1]; v.w = pim[e + 1];
+        *reinterpret_cast<float4 *>(a + addr) = v;
+    }
+}
+
+// Host-side: operand-ready matrices for a block.  u: d x d complex doubles
+// (row bit j = bit(qubits[j])).  out: (split ? 2 : 1) * HALVES * D * 64 floats.
+void fused_operands(int kq, const double *u, int split, float *out)
+{
+    const int d = 1 << kq, R = 2 * d, halves = R / 32;
+    auto M = [&](int i, int j) -> double {
+        const int r = i >> 1, c = j >> 1;
+        const double ur = u[2 * (r * d + c)], ui = u[2 * (r * d + c) + 1];
+        if ((i & 1) == 0) return (j & 1) == 0 ? ur : -ui;
+        return (j & 1) == 0 ? ui : ur;
+    };
+    for (int h = 0; h < halves; h++)
+        for (int s = 0; s < d; s++)
+            for (int l = 0; l < 64; l++) {
+                const int i = 32 * h + (l & 31), j = 2 * s + (l >> 5);
+                const double v = M(i, j);
+                const float hi = (float)v;
+                out[(h * d + s) * 64 + l] = hi;
+                if (split) out[(halves * d + h * d + s) * 64 + l] = (float)(v - (double)hi);
+            }
+}
+
+size_t fused_operand_floats(int kq, int split)
+{
+    const int d = 1 << kq;
+    return (size_t)(split ? 2 : 1) * ((2 * d) / 32) * d * 64;
+}
+
+// Launch one fused block (kq = 4 or 5, qubits[0..kq-1] distinct) with
+// operands already on the device.
+int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split)
+{
+    if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused MFMA blocks need a c64 state");
+    if (kq != 4 && kq != 5) return fail(QSV_ERR_ARG, "fused block size must be 4 or 5");
+    if (s->n < kq + 7) return fail(QSV_ERR_ARG, "state too small for a fused tile");
+    FusedArgs fa{};
+    uint64_t qmask = 0;
+    for (int j = 0; j < kq; j++) qmask |= uint64_t(1) << qubits[j];
+    // the 7 lowest free bits
+    int free_bits[7], nf = 0;
+    for (int b = 0; b < s->n && nf < 7; b++)
+        if (!(qmask >> b & 1)) free_bits[nf++] = b;
+    uint64_t lmask = qmask;
+    for (int i = 0; i < 7; i++) lmask |= uint64_t(1) << free_bits[i];
+    fa.tile_ins = insert_set_from_mask(lmask);
+    int lpos[12];
+    for (int i = 0; i < fa.tile_ins.n; i++) lpos[i] = fa.tile_ins.pos[i];
+    auto rank = [&](int bit) {
+        for (int i = 0; i < kq + 7; i++)
+            if (lpos[i] == bit) return i;
+        return -1;
+    };
+    for (int i = 0; i < 7; i++) fa.col_rank[i] = rank(free_bits[i]);
+    const int d = 1 << kq;
+    for (int r = 0; r < d; r++) {
+        int e = 0;
+        for (int j = 0; j < kq; j++)
+            if (r >> j & 1) e |= 1 << rank(qubits[j]);
+        fa.erow[r] = e;
+    }
+    // L positions 0..6 are qubits 0..6; ranks >= 7 map the high tile bits
+    for (int hbits = 0; hbits < d; hbits++) {
+        uint64_t o = 0;
+        for (int j = 0; j < kq; j++)
+            if (hbits >> j & 1) o |= uint64_t(1) << lpos[7 + j];
+        fa.off_high[hbits] = o;
+    }
+    const uint64_t ntiles = s->dim >> (kq + 7);
+    float2 *a = (float2 *)s->amps;
+    if (kq == 4) {
+        if (split) hipLaunchKernelGGL((k_fused_mfma<4, true>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
+        else hipLaunchKernelGGL((k_fused_mfma<4, false>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
+    } else {
+        if (split) hipLaunchKernelGGL((k_fused_mfma<5, true>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
+        else hipLaunchKernelGGL((k_fused_mfma<5, false>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
+    }
+    s->launches++;
+    QSV_HIP(hipGetLastError());
+    return QSV_OK;
+}
+
+// qsv_apply_block on a c64 state: blocks of 4 or 5 qubits go to the MFMA
+// kernel; smaller blocks are widened with identity on the lowest free qubits
+// (same sweep, same kernel); states too small for a tile use the FP64 block
+// kernel.
+int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int split)
+{
+    int kq = k < 4 ? 4 : k;
+    if (s->n < kq + 7) return launch_block(s, qubits, k, u, 0);
+    std::vector<int> qs(qubits, qubits + k);
+    std::vector<double> w;
+    const double *uu = u;
+    if (kq != k) {
+        uint64_t used = 0;
+        for (int q : qs) used |= uint64_t(1) << q;
+        for (int b = 0; (int)qs.size() < kq; b++)
+            if (!(used >> b & 1)) qs.push_back(b);
+        const int d = 1 << k, D = 1 << kq;
+        w.assign(2 * (size_t)D * D, 0.0);
+        // U (x) I: rows/cols with equal high (padding) bits carry U
+        for (int hi = 0; hi < D / d; hi++)
+            for (int r = 0; r < d; r++)
+                for (int c = 0; c < d; c++) {
+                    const size_t R = (size_t)hi * d + r, Cc = (size_t)hi * d + c;
+                    w[2 * (R * D + Cc)] = u[2 * (r * d + c)];
+                    w[2 * (R * D + Cc) + 1] = u[2 * (r * d + c) + 1];
+                }
+        uu = w.data();
+    }
+    std::vector<float> ops(fused_operand_floats(kq, split));
+    fused_operands(kq, uu, split, ops.data());
+    const size_t bytes = ops.size() * sizeof(float);
+    if (bytes > sizeof(double) * (kRedSlots - 8)) return fail(QSV_ERR_ARG, "operand block too large");
+    QSV_HIP(hipMemcpyAsync(s->red, ops.data(), bytes, hipMemcpyHostToDevice, s->stream));
+    int rc = launch_fused_dev(s, qs.data(), kq, (const float *)s->red, split);
+    if (!rc) QSV_HIP(hipStreamSynchronize(s->stream));   // ops is a host temporary
+    return rc;
 }
 
 }  // namespace qsv

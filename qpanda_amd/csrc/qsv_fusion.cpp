@@ -1,12 +1,437 @@
-// qsv_fusion.cpp -- placeholder until the block scheduler lands.
+// qsv_fusion.cpp -- fusing a gate list into dense k-qubit blocks.
+//
+// Greedy planner.  Every qubit has at most one "open" block (the latest block
+// touching it).  For each gate with qubit set S:
+//   * the open blocks touching S are candidates for merging; the planner
+//     keeps the subset that fits in max_k qubits together with S and holds the
+//     most gates, closes the others, merges the kept ones with the gate;
+//   * a block that changes is moved to the end of the emission order.  That is
+//     always legal: a block is open on all of its qubits, so every block
+//     emitted after it acts on other qubits and commutes with it.
+// MEASURE and RESET records close every block and run on their own.
+//
+// Block matrices are accumulated in FP64 on the host (gate matrices follow
+// the CPU simulator's conventions, CPUQuantumGates.cpp); the device sees one
+// FP32 MFMA sweep per block (qsv_fused.hip).  Small blocks are widened to 4
+// qubits with identity on the lowest free qubits.
 #include "qsv_fusion.h"
+#include "qsv_sched.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
 
 namespace qsv {
 
-int run_fused(qsv_state *, const qsv_gate *, int64_t, const double *, const qsv_run_opts &, int32_t *,
-              int *, int64_t *, double *)
+using cd = std::complex<double>;
+
+namespace {
+
+const double kSq2Ref = 0.707106781186548;
+const double kSq2Exact = 0.70710678118654752440;
+
+// Local matrix of a gate: qs[j] is the qubit of row bit j.
+int gate_matrix(const qsv_gate &r, const double *mats, int exact, int *qs, std::vector<cd> &m)
 {
-    return fail(QSV_ERR_UNSUPPORTED, "fused path not built");
+    auto set2 = [&](cd a, cd b, cd c, cd d) { m = {a, b, c, d}; };
+    const double h = exact ? kSq2Exact : kSq2Ref;
+    switch (r.kind) {
+    case QSV_GATE_H: qs[0] = r.q0; set2(h, h, h, -h); return 1;
+    case QSV_GATE_X: qs[0] = r.q0; set2(0, 1, 1, 0); return 1;
+    case QSV_GATE_RX: {
+        qs[0] = r.q0;
+        const double c = std::cos(r.theta / 2), s = std::sin(r.theta / 2);
+        const cd k(0, r.dagger ? s : -s);
+        set2(c, k, k, c);
+        return 1;
+    }
+    case QSV_GATE_RY: {
+        qs[0] = r.q0;
+        const double c = std::cos(r.theta / 2), s = std::sin(r.theta / 2), g = r.dagger ? -1 : 1;
+        set2(c, -g * s, g * s, c);
+        return 1;
+    }
+    case QSV_GATE_RZ: {
+        qs[0] = r.q0;
+        const double c = std::cos(r.theta / 2), s = std::sin(r.theta / 2), s1 = r.dagger ? -s : s;
+        set2(cd(c, -s1), 0, 0, cd(c, s1));
+        return 1;
+    }
+    case QSV_GATE_U1: {
+        qs[0] = r.q0;
+        const double *u = mats + 2 * (size_t)r.mat;
+        set2(cd(u[0], u[1]), cd(u[2], u[3]), cd(u[4], u[5]), cd(u[6], u[7]));
+        return 1;
+    }
+    case QSV_GATE_CNOT:
+        if (r.q0 == r.q1) return 0;
+        qs[0] = r.q1; qs[1] = r.q0;        // bit0 = target, bit1 = control
+        m.assign(16, 0);
+        m[0 * 4 + 0] = m[1 * 4 + 1] = 1;
+        m[2 * 4 + 3] = m[3 * 4 + 2] = 1;
+        return 2;
+    case QSV_GATE_CZ: case QSV_GATE_CR: {
+        if (r.q0 == r.q1) return 0;
+        qs[0] = r.q0; qs[1] = r.q1;
+        m.assign(16, 0);
+        m[0] = m[5] = m[10] = 1;
+        if (r.kind == QSV_GATE_CZ) m[15] = -1;
+        else m[15] = cd(std::cos(r.theta), r.dagger ? -std::sin(r.theta) : std::sin(r.theta));
+        return 2;
+    }
+    case QSV_GATE_ISWAP: case QSV_GATE_SQISWAP: {
+        if (r.q0 == r.q1) return 0;
+        qs[0] = r.q0; qs[1] = r.q1;
+        m.assign(16, 0);
+        m[0] = m[15] = 1;
+        const cd k(0, r.dagger ? -1 : 1);
+        if (r.kind == QSV_GATE_ISWAP) {
+            m[1 * 4 + 2] = m[2 * 4 + 1] = k;
+        } else {
+            m[1 * 4 + 1] = m[2 * 4 + 2] = h;
+            m[1 * 4 + 2] = m[2 * 4 + 1] = k * h;
+        }
+        return 2;
+    }
+    case QSV_GATE_U2: {
+        qs[0] = r.q1; qs[1] = r.q0;        // row = 2*bit(q0) + bit(q1)
+        const double *u = mats + 2 * (size_t)r.mat;
+        m.resize(16);
+        for (int i = 0; i < 16; i++) m[i] = cd(u[2 * i], u[2 * i + 1]);
+        return 2;
+    }
+    case QSV_GATE_TOFFOLI:
+        qs[0] = r.q2; qs[1] = r.q0; qs[2] = r.q1;   // bit0 = target
+        m.assign(64, 0);
+        for (int i = 0; i < 6; i++) m[i * 8 + i] = 1;
+        m[6 * 8 + 7] = m[7 * 8 + 6] = 1;
+        return 3;
+    case QSV_GATE_CSWAP:
+        qs[0] = r.q0; qs[1] = r.q1; qs[2] = r.q2;   // bit0 = control
+        m.assign(64, 0);
+        for (int i = 0; i < 8; i++) m[i * 8 + i] = 1;
+        m[3 * 8 + 3] = m[5 * 8 + 5] = 0;
+        m[3 * 8 + 5] = m[5 * 8 + 3] = 1;
+        return 3;
+    default:
+        return -1;
+    }
+}
+
+// Apply an m-qubit matrix g (on block positions pos[0..m-1]) to every column
+// of the block matrix B (dimension 2^nq): B <- G B.
+void apply_small(std::vector<cd> &B, int nq, const int *pos, int m, const std::vector<cd> &g)
+{
+    const int D = 1 << nq, d = 1 << m;
+    int mask = 0;
+    for (int j = 0; j < m; j++) mask |= 1 << pos[j];
+    std::vector<int> off(d);
+    for (int r = 0; r < d; r++) {
+        int o = 0;
+        for (int j = 0; j < m; j++)
+            if (r >> j & 1) o |= 1 << pos[j];
+        off[r] = o;
+    }
+    std::vector<cd> x(d);
+    for (int col = 0; col < D; col++)
+        for (int base = 0; base < D; base++) {
+            if (base & mask) continue;
+            for (int c = 0; c < d; c++) x[c] = B[(size_t)(base | off[c]) * D + col];
+            for (int r = 0; r < d; r++) {
+                cd acc = 0;
+                for (int c = 0; c < d; c++) acc += g[(size_t)r * d + c] * x[c];
+                B[(size_t)(base | off[r]) * D + col] = acc;
+            }
+        }
+}
+
+struct Block {
+    std::vector<int> qs;
+    std::vector<cd> m;
+    int64_t seq = 0;
+    int64_t ngates = 0;
+    bool open = true;
+    bool alive = true;
+    int64_t barrier = -1;      // record index for MEASURE/RESET items
+};
+
+int index_of(const std::vector<int> &v, int q)
+{
+    for (size_t i = 0; i < v.size(); i++)
+        if (v[i] == q) return (int)i;
+    return -1;
+}
+
+// Re-express block b on the qubit list `qs` (a superset of b.qs).
+void widen(Block &b, const std::vector<int> &qs)
+{
+    if (qs == b.qs) return;
+    const int D = 1 << qs.size();
+    std::vector<cd> I((size_t)D * D, 0);
+    for (int i = 0; i < D; i++) I[(size_t)i * D + i] = 1;
+    if (!b.qs.empty()) {
+        std::vector<int> pos(b.qs.size());
+        for (size_t j = 0; j < b.qs.size(); j++) pos[j] = index_of(qs, b.qs[j]);
+        apply_small(I, (int)qs.size(), pos.data(), (int)b.qs.size(), b.m);
+    }
+    b.qs = qs;
+    b.m.swap(I);
+}
+
+}  // namespace
+
+int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
+                std::vector<PlanItem> &plan)
+{
+    if (max_k < 1 || max_k > 5) return fail(QSV_ERR_ARG, "fusion size must be 1..5");
+    std::vector<Block> blocks;
+    int cur[64];
+    for (int &c : cur) c = -1;
+    int64_t clock = 0;
+    auto close = [&](int bi) {
+        Block &b = blocks[bi];
+        if (!b.open) return;
+        b.open = false;
+        for (int q : b.qs)
+            if (cur[q] == bi) cur[q] = -1;
+    };
+    for (int64_t gi = 0; gi < ng; gi++) {
+        const qsv_gate &r = gates[gi];
+        if (r.kind == QSV_GATE_MEASURE || r.kind == QSV_GATE_RESET) {
+            for (size_t bi = 0; bi < blocks.size(); bi++) close((int)bi);
+            Block b;
+            b.barrier = gi;
+            b.seq = ++clock;
+            b.open = false;
+            blocks.push_back(b);
+            continue;
+        }
+        int qs[3];
+        std::vector<cd> gm;
+        const int nq = gate_matrix(r, mats, exact, qs, gm);
+        if (nq < 0) return fail(QSV_ERR_UNSUPPORTED, "gate kind cannot be fused");
+        if (nq == 0) continue;                       // no-op record (e.g. CNOT(q, q))
+        // candidate blocks: open blocks on the gate's qubits
+        std::vector<int> cand;
+        for (int j = 0; j < nq; j++)
+            if (cur[qs[j]] >= 0 && std::find(cand.begin(), cand.end(), cur[qs[j]]) == cand.end())
+                cand.push_back(cur[qs[j]]);
+        // best subset that fits with the gate
+        int best = -1;
+        int64_t best_gates = -1;
+        size_t best_width = 99;
+        for (int sub = 0; sub < (1 << cand.size()); sub++) {
+            std::vector<int> u(qs, qs + nq);
+            int64_t gcount = 0;
+            for (size_t c = 0; c < cand.size(); c++)
+                if (sub >> c & 1) {
+                    for (int q : blocks[cand[c]].qs)
+                        if (index_of(u, q) < 0) u.push_back(q);
+                    gcount += blocks[cand[c]].ngates;
+                }
+            if ((int)u.size() > std::max(max_k, nq)) continue;   // a wide gate may stand alone
+            if (gcount > best_gates || (gcount == best_gates && u.size() < best_width)) {
+                best = sub;
+                best_gates = gcount;
+                best_width = u.size();
+            }
+        }
+        if (best < 0) return fail(QSV_ERR_ARG, "gate wider than the fusion size");
+        std::vector<int> keep;
+        for (size_t c = 0; c < cand.size(); c++) {
+            if (best >> c & 1) keep.push_back(cand[c]);
+            else close(cand[c]);
+        }
+        // target block: the first kept one, or a new block
+        int ti;
+        if (keep.empty()) {
+            Block b;
+            blocks.push_back(b);
+            ti = (int)blocks.size() - 1;
+        } else {
+            ti = keep[0];
+        }
+        std::vector<int> u = blocks[ti].qs;
+        for (size_t c = 1; c < keep.size(); c++)
+            for (int q : blocks[keep[c]].qs)
+                if (index_of(u, q) < 0) u.push_back(q);
+        for (int j = 0; j < nq; j++)
+            if (index_of(u, qs[j]) < 0) u.push_back(qs[j]);
+        widen(blocks[ti], u);
+        for (size_t c = 1; c < keep.size(); c++) {
+            Block &o = blocks[keep[c]];
+            std::vector<int> pos(o.qs.size());
+            for (size_t j = 0; j < o.qs.size(); j++) pos[j] = index_of(u, o.qs[j]);
+            apply_small(blocks[ti].m, (int)u.size(), pos.data(), (int)o.qs.size(), o.m);
+            blocks[ti].ngates += o.ngates;
+            o.alive = false;
+            o.open = false;
+        }
+        int pos[3];
+        for (int j = 0; j < nq; j++) pos[j] = index_of(u, qs[j]);
+        apply_small(blocks[ti].m, (int)u.size(), pos, nq, gm);
+        Block &t = blocks[ti];
+        t.ngates += 1;
+        t.seq = ++clock;
+        t.open = true;
+        t.alive = true;
+        for (int q : t.qs) cur[q] = ti;
+    }
+    std::vector<int> order;
+    for (size_t i = 0; i < blocks.size(); i++)
+        if (blocks[i].alive) order.push_back((int)i);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return blocks[x].seq < blocks[y].seq; });
+    plan.clear();
+    for (int i : order) {
+        PlanItem it;
+        it.barrier = blocks[i].barrier;
+        it.qubits = blocks[i].qs;
+        it.ngates = blocks[i].ngates;
+        it.u.resize(2 * blocks[i].m.size());
+        for (size_t j = 0; j < blocks[i].m.size(); j++) {
+            it.u[2 * j] = blocks[i].m[j].real();
+            it.u[2 * j + 1] = blocks[i].m[j].imag();
+        }
+        plan.push_back(std::move(it));
+    }
+    return QSV_OK;
+}
+
+// Widen a block to `kq` qubits with identity on the lowest qubits it does not
+// use (U (x) I, the new qubits taking the high row bits).
+void widen_to(PlanItem &it, int kq)
+{
+    const int k = (int)it.qubits.size();
+    if (k >= kq) return;
+    uint64_t used = 0;
+    for (int q : it.qubits) used |= uint64_t(1) << q;
+    std::vector<int> qs = it.qubits;
+    for (int b = 0; (int)qs.size() < kq; b++)
+        if (!(used >> b & 1)) qs.push_back(b);
+    const int d = 1 << k, D = 1 << kq;
+    std::vector<double> w(2 * (size_t)D * D, 0.0);
+    for (int hi = 0; hi < D / d; hi++)
+        for (int r = 0; r < d; r++)
+            for (int c = 0; c < d; c++) {
+                const size_t R = (size_t)hi * d + r, C = (size_t)hi * d + c;
+                w[2 * (R * D + C)] = it.u[2 * ((size_t)r * d + c)];
+                w[2 * (R * D + C) + 1] = it.u[2 * ((size_t)r * d + c) + 1];
+            }
+    it.qubits = qs;
+    it.u.swap(w);
+}
+
+int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
+              const qsv_run_opts &o, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms,
+              Profiler *prof)
+{
+    if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "the fused path runs on c64 states");
+    std::vector<PlanItem> plan;
+    int rc;
+    SchedChoice choice;
+    if (o.scheduler == 1) {
+        rc = plan_learned(s, gates, ngates, mats, o, plan, &choice);
+    } else {
+        rc = plan_greedy(gates, ngates, mats, std::min(o.fuse_k, 5), o.exact, plan);
+        choice.fuse_k = o.fuse_k;
+    }
+    if (rc) return rc;
+    const int kq_min = 4;
+    const bool tiles_fit = s->n >= kq_min + 7;
+    // operands for every block in one upload
+    std::vector<size_t> op_off(plan.size(), 0);
+    std::vector<float> ops;
+    for (size_t i = 0; i < plan.size(); i++) {
+        PlanItem &it = plan[i];
+        if (it.barrier >= 0) continue;
+        if (!tiles_fit) continue;
+        widen_to(it, kq_min);
+        const int kq = (int)it.qubits.size();
+        op_off[i] = ops.size();
+        ops.resize(ops.size() + fused_operand_floats(kq, o.split));
+        fused_operands(kq, it.u.data(), o.split, ops.data() + op_off[i]);
+    }
+    float *dops = nullptr;
+    if (!ops.empty()) {
+        if (hipMalloc((void **)&dops, ops.size() * sizeof(float)) != hipSuccess)
+            return fail(QSV_ERR_ALLOC, "cannot allocate block operands");
+        hipError_t e = hipMemcpyAsync(dops, ops.data(), ops.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(dops);
+            return hip_fail(e, "operand upload");
+        }
+    }
+    const double N = (double)s->dim;
+    double pred = 0;
+    int64_t nblocks = 0;
+    rc = QSV_OK;
+    for (size_t i = 0; i < plan.size() && !rc; i++) {
+        const PlanItem &it = plan[i];
+        if (it.barrier >= 0) {
+            const qsv_gate &r = gates[it.barrier];
+            // one draw for the measurement, one more for a Reset's NOT on outcome 1
+            double draw = 1.0;
+            if (rng) {
+                *rng = qsv_pm_next(*rng);
+                draw = (double)*rng / 2147483647;
+            }
+            int out = 0;
+            if ((rc = prof->begin())) break;
+            rc = qsv_measure(s, r.q0, draw, &out);
+            if (!rc && r.kind == QSV_GATE_MEASURE && outcomes) outcomes[r.q1] = out;
+            if (!rc && r.kind == QSV_GATE_RESET && out) {
+                if (rng) *rng = qsv_pm_next(*rng);
+                rc = qsv_x(s, r.q0, 0);
+            }
+            if (!rc) rc = prof->end(QSV_K_READOUT, 2 * 8 * N, 0);
+            continue;
+        }
+        if (rng) *rng = qsv_pm_jump(*rng, (uint64_t)it.ngates);
+        const int kq = (int)it.qubits.size();
+        if ((rc = prof->begin())) break;
+        if (tiles_fit) {
+            rc = launch_fused_dev(s, it.qubits.data(), kq, dops + op_off[i], o.split);
+            if (!rc) rc = prof->end(QSV_K_FUSED, 2 * 8 * N, 8.0 * N * (1 << kq) * (o.split ? 2 : 1));
+        } else {
+            rc = launch_block(s, it.qubits.data(), kq, it.u.data(), 0);
+            if (!rc) rc = prof->end(QSV_K_BLOCK64, 2 * 8 * N, 8.0 * N * (1 << kq));
+        }
+        pred += it.pred_ms;
+        nblocks++;
+    }
+    if (dops) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(dops);
+    }
+    *blocks = nblocks;
+    *pred_ms = pred;
+    return rc;
 }
 
 }  // namespace qsv
+
+using namespace qsv;
+
+extern "C" int qsv_fuse_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
+                             int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                             int64_t *ngates_out, int64_t *nblocks)
+{
+    if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
+    std::vector<PlanItem> plan;
+    int rc = plan_greedy(gates, ngates, mats, max_k, exact, plan);
+    if (rc) return rc;
+    *nblocks = (int64_t)plan.size();
+    if ((int64_t)plan.size() > max_blocks) return fail(QSV_ERR_ARG, "plan larger than the output buffers");
+    for (size_t b = 0; b < plan.size(); b++) {
+        const PlanItem &it = plan[b];
+        const int k = it.barrier >= 0 ? 0 : (int)it.qubits.size();
+        if (k_out) k_out[b] = k;
+        if (ngates_out) ngates_out[b] = it.barrier >= 0 ? it.barrier : it.ngates;
+        if (qubits_out)
+            for (int j = 0; j < 5; j++) qubits_out[b * 5 + j] = j < k ? it.qubits[j] : -1;
+        if (mats_out && k > 0) std::memcpy(mats_out + b * 2048, it.u.data(), it.u.size() * sizeof(double));
+    }
+    return QSV_OK;
+}

@@ -7,10 +7,30 @@ namespace qsv {
 
 int validate_gates(const qsv_state *s, const qsv_gate *g, int64_t ng);
 int upload_mats(qsv_state *s, const double *mats, size_t count_complex);
+int record_class(const qsv_state *s, const qsv_gate &r, double *bytes, double *flops);
+
+// Per-launch event timing on the state's stream: begin() opens the first
+// interval, each end() closes one launch and opens the next.
+class Profiler {
+public:
+    Profiler(qsv_state *s, bool on);
+    ~Profiler();
+    int begin();
+    int end(int cls, double bytes, double flops);
+    int collect(qsv_run_stats *st);
+    bool on() const { return on_; }
+
+private:
+    qsv_state *s_;
+    bool on_;
+    std::vector<hipEvent_t> events_;
+    std::vector<int> cls_;
+    std::vector<double> bytes_, flops_;
+};
 
 // Fuse `gates` into blocks of at most opts.fuse_k qubits and apply them.
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &opts, int32_t *rng, int *outcomes, int64_t *blocks,
-              double *pred_ms);
+              double *pred_ms, Profiler *prof);
 
 }  // namespace qsv

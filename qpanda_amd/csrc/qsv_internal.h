@@ -96,6 +96,8 @@ int reduce_prob0(qsv_state *s, int q, double *p0);
 int reduce_norm2(qsv_state *s, double *v);
 int reduce_zz_chain(qsv_state *s, double *v);
 int serial_prob0(qsv_state *s, int q, double *p0);
+int inner(qsv_state *a, qsv_state *b, double *re, double *im);
+int copy_state(qsv_state *d, qsv_state *s);
 int launch_collapse(qsv_state *s, int q, int outcome, double scale);
 int marginals(qsv_state *s, const int *qubits, int m, double *out);
 

@@ -115,6 +115,63 @@ __global__ __launch_bounds__(kBlock) void k_collapse(T *__restrict__ a, uint64_t
     a[drop] = z;
 }
 
+// <a|b> over two states of possibly different precisions: per-block partial
+// (re, im) pairs in a fixed order, then the same fixed final pass.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_inner(const TA *__restrict__ a, const TB *__restrict__ b,
+                                                   uint64_t count, uint64_t chunk, double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock / 64];
+    const uint64_t start = (uint64_t)blockIdx.x * chunk;
+    uint64_t end = start + chunk;
+    if (end > count) end = count;
+    double re = 0, im = 0;
+    for (uint64_t i = start + threadIdx.x; i < end; i += kBlock) {
+        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        re += ar * br + ai * bi;          // conj(a) * b
+        im += ar * bi - ai * br;
+    }
+    const double tr = block_sum(re, lds);
+    __syncthreads();
+    const double ti = block_sum(im, lds);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = tr;
+        partial[2 * blockIdx.x + 1] = ti;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_inner_final(const double *__restrict__ partial, int n,
+                                                         double *__restrict__ out)
+{
+    __shared__ double lds[kBlock / 64];
+    double re = 0, im = 0;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        re += partial[2 * i];
+        im += partial[2 * i + 1];
+    }
+    const double tr = block_sum(re, lds);
+    __syncthreads();
+    const double ti = block_sum(im, lds);
+    if (threadIdx.x == 0) {
+        out[0] = tr;
+        out[1] = ti;
+    }
+}
+
+template <typename TD, typename TS>
+__global__ __launch_bounds__(kBlock) void k_copy(TD *__restrict__ d, const TS *__restrict__ s, uint64_t count)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    using R = decltype(TD{}.x);
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const TS v = s[i];
+        TD w;
+        w.x = (R)v.x;
+        w.y = (R)v.y;
+        d[i] = w;
+    }
+}
+
 // ---- marginals --------------------------------------------------------------
 constexpr int kTileBits = 12;
 
@@ -223,6 +280,58 @@ int reduce_common(qsv_state *s, int mode, int q, double *result)
 int reduce_prob0(qsv_state *s, int q, double *p0) { return reduce_common(s, RED_PROB0, q, p0); }
 int reduce_norm2(qsv_state *s, double *v) { return reduce_common(s, RED_NORM, 0, v); }
 int reduce_zz_chain(qsv_state *s, double *v) { return reduce_common(s, RED_ZZ, 0, v); }
+
+int inner(qsv_state *a, qsv_state *b, double *re, double *im)
+{
+    const uint64_t count = a->dim;
+    uint64_t nblk = (count + (uint64_t)kBlock * 16 - 1) / ((uint64_t)kBlock * 16);
+    if (nblk > 2048) nblk = 2048;
+    if (nblk == 0) nblk = 1;
+    uint64_t chunk = (count + nblk - 1) / nblk;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    nblk = (count + chunk - 1) / chunk;
+    QSV_HIP(hipStreamSynchronize(b->stream));           // b's pending gates first
+    double *partial = a->red;
+    double *out = a->red + (kRedSlots - 4);
+    const dim3 g((unsigned)nblk), t(kBlock);
+    if (a->prec == QSV_C128 && b->prec == QSV_C128)
+        hipLaunchKernelGGL((k_inner<c128, c128>), g, t, 0, a->stream, (const c128 *)a->amps, (const c128 *)b->amps, count, chunk, partial);
+    else if (a->prec == QSV_C128)
+        hipLaunchKernelGGL((k_inner<c128, c64>), g, t, 0, a->stream, (const c128 *)a->amps, (const c64 *)b->amps, count, chunk, partial);
+    else if (b->prec == QSV_C128)
+        hipLaunchKernelGGL((k_inner<c64, c128>), g, t, 0, a->stream, (const c64 *)a->amps, (const c128 *)b->amps, count, chunk, partial);
+    else
+        hipLaunchKernelGGL((k_inner<c64, c64>), g, t, 0, a->stream, (const c64 *)a->amps, (const c64 *)b->amps, count, chunk, partial);
+    hipLaunchKernelGGL(k_inner_final, dim3(1), t, 0, a->stream, partial, (int)nblk, out);
+    a->launches += 2;
+    QSV_HIP(hipGetLastError());
+    QSV_HIP(hipMemcpyAsync(a->host_red, out, 2 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    QSV_HIP(hipStreamSynchronize(a->stream));
+    *re = a->host_red[0];
+    *im = a->host_red[1];
+    return QSV_OK;
+}
+
+int copy_state(qsv_state *d, qsv_state *s)
+{
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    uint64_t blocks = (d->dim + kBlock * 8 - 1) / (kBlock * 8);
+    if (blocks > 65536) blocks = 65536;
+    if (blocks == 0) blocks = 1;
+    const dim3 g((unsigned)blocks), t(kBlock);
+    if (d->prec == QSV_C128 && s->prec == QSV_C128)
+        hipLaunchKernelGGL((k_copy<c128, c128>), g, t, 0, d->stream, (c128 *)d->amps, (const c128 *)s->amps, d->dim);
+    else if (d->prec == QSV_C128)
+        hipLaunchKernelGGL((k_copy<c128, c64>), g, t, 0, d->stream, (c128 *)d->amps, (const c64 *)s->amps, d->dim);
+    else if (s->prec == QSV_C128)
+        hipLaunchKernelGGL((k_copy<c64, c128>), g, t, 0, d->stream, (c64 *)d->amps, (const c128 *)s->amps, d->dim);
+    else
+        hipLaunchKernelGGL((k_copy<c64, c64>), g, t, 0, d->stream, (c64 *)d->amps, (const c64 *)s->amps, d->dim);
+    d->launches++;
+    QSV_HIP(hipGetLastError());
+    QSV_HIP(hipStreamSynchronize(d->stream));
+    return QSV_OK;
+}
 
 int serial_prob0(qsv_state *s, int q, double *p0)
 {

@@ -1,0 +1,42 @@
+// qsv_sched.h -- fusion plans, fused-block operands and the learned scheduler.
+#pragma once
+
+#include "qsv_internal.h"
+
+#include <vector>
+
+namespace qsv {
+
+class Profiler;
+
+// One item of a fusion plan: a dense block (qubits, matrix) or a barrier
+// record (MEASURE / RESET) that runs on its own.
+struct PlanItem {
+    int64_t barrier = -1;          // record index, or -1 for a block
+    std::vector<int> qubits;       // row bit j = bit(qubits[j])
+    std::vector<double> u;         // 2^k x 2^k complex, row-major (re, im)
+    int64_t ngates = 0;            // logical gates in the block
+    int tile = 0;                  // tile-shape choice (0 = default)
+    double pred_ms = 0;            // predicted device time
+};
+
+struct SchedChoice {
+    int fuse_k = 4;
+    double pred_ms = 0;
+};
+
+int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
+                std::vector<PlanItem> &plan);
+void widen_to(PlanItem &it, int kq);
+
+// Learned scheduler (qsv_costmodel.hip): plans at every k, predicts each
+// block's time with the cost model, keeps the cheapest plan.
+int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *mats,
+                 const qsv_run_opts &o, std::vector<PlanItem> &plan, SchedChoice *choice);
+
+// Fused MFMA kernel helpers (qsv_fused.hip).
+void fused_operands(int kq, const double *u, int split, float *out);
+size_t fused_operand_floats(int kq, int split);
+int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split);
+
+}  // namespace qsv

@@ -25,7 +25,7 @@ EXPORTED = [
     "qsv_apply_block", "qsv_prob0", "qsv_norm2", "qsv_collapse", "qsv_measure",
     "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
     "qsv_sample_shots", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
-    "qsv_last_error", "qsv_device_count",
+    "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan",
 ]
 
 
@@ -35,13 +35,27 @@ class QsvError(RuntimeError):
 
 class RunOpts(C.Structure):
     _fields_ = [("exact", C.c_int), ("fuse_k", C.c_int), ("split", C.c_int),
-                ("scheduler", C.c_int), ("reserved", C.c_int * 4)]
+                ("scheduler", C.c_int), ("profile", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class RunStats(C.Structure):
     _fields_ = [("gates", C.c_int64), ("launches", C.c_int64), ("blocks", C.c_int64),
                 ("tie_resolutions", C.c_int64), ("device_ms", C.c_double),
-                ("model_ms_pred", C.c_double)]
+                ("model_ms_pred", C.c_double), ("class_launches", C.c_int64 * 8),
+                ("class_ms", C.c_double * 8), ("class_bytes", C.c_double * 8),
+                ("class_flops", C.c_double * 8)]
+
+    CLASS_NAMES = ["dense1", "diag", "perm", "phase", "block64", "fused", "readout", "other"]
+
+    def classes(self) -> dict:
+        out = {}
+        for i, nm in enumerate(self.CLASS_NAMES):
+            if self.class_launches[i]:
+                ms = self.class_ms[i]
+                out[nm] = {"launches": int(self.class_launches[i]), "ms": ms,
+                           "bytes": self.class_bytes[i], "flops": self.class_flops[i],
+                           "gbs": self.class_bytes[i] / (ms * 1e6) if ms > 0 else 0.0}
+        return out
 
 
 _lib = None
@@ -99,6 +113,10 @@ def lib():
             "qsv_version": ([], C.c_char_p),
             "qsv_last_error": ([], C.c_char_p),
             "qsv_device_count": ([C.POINTER(I)], I),
+            "qsv_fuse_plan": ([vp, C.c_int64, vp, I, I, C.c_int64, vp, vp, vp, vp,
+                               C.POINTER(C.c_int64)], I),
+            "qsv_copy_state": ([vp, vp], I),
+            "qsv_inner": ([vp, vp, C.POINTER(D), C.POINTER(D)], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -169,6 +187,19 @@ class StateVector:
 
     def sync(self):
         _check(lib().qsv_sync(self.h))
+
+    def copy_from(self, other: "StateVector"):
+        _check(lib().qsv_copy_state(self.h, other.h))
+
+    def inner(self, other: "StateVector") -> complex:
+        re, im = C.c_double(), C.c_double()
+        _check(lib().qsv_inner(self.h, other.h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def fidelity(self, other: "StateVector") -> float:
+        """|<self|other>|^2 / (<self|self> <other|other>), all in FP64 on the device."""
+        z = self.inner(other)
+        return abs(z) ** 2 / (self.norm2() * other.norm2())
 
     def device_ptr(self) -> int:
         p = C.c_void_p()
@@ -247,10 +278,10 @@ class StateVector:
         return out, rs.value, ties.value
 
     def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=True, scheduler=0,
-            rng_state=None, outcomes=None) -> RunStats:
+            rng_state=None, outcomes=None, profile=False) -> RunStats:
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
-        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler))
+        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile))
         st = RunStats()
         rs = C.c_int32(rng_state if rng_state is not None else 1)
         self._last_rng = None
@@ -265,6 +296,35 @@ class StateVector:
     @property
     def last_rng(self):
         return self._last_rng
+
+
+def fuse_plan(circ: "_circ.Circuit", max_k: int, exact: bool = False):
+    """Greedy fusion plan (host code; no device needed).  Returns a list of
+    (qubits, matrix, ngates) blocks, or (None, None, record_index) for
+    MEASURE/RESET records, in emission order."""
+    g = np.ascontiguousarray(circ.gates)
+    m = np.ascontiguousarray(circ.mats, np.complex128)
+    cap = max(1, g.shape[0])
+    ks = np.zeros(cap, np.int32)
+    qs = np.zeros(cap * 5, np.int32)
+    mats = np.zeros(cap * 2048, np.float64)
+    ngs = np.zeros(cap, np.int64)
+    nb = C.c_int64()
+    _check(lib().qsv_fuse_plan(g.ctypes.data_as(C.c_void_p), g.shape[0],
+                               m.ctypes.data_as(C.c_void_p) if m.size else None, max_k, int(exact), cap,
+                               ks.ctypes.data_as(C.c_void_p), qs.ctypes.data_as(C.c_void_p),
+                               mats.ctypes.data_as(C.c_void_p), ngs.ctypes.data_as(C.c_void_p),
+                               C.byref(nb)))
+    out = []
+    for b in range(nb.value):
+        k = int(ks[b])
+        if k == 0:
+            out.append((None, None, int(ngs[b])))
+            continue
+        d = 1 << k
+        u = mats[b * 2048: b * 2048 + 2 * d * d].view(np.complex128).reshape(d, d).copy()
+        out.append((qs[b * 5: b * 5 + k].tolist(), u, int(ngs[b])))
+    return out
 
 
 def device_count() -> int:

@@ -1,0 +1,137 @@
+"""Fused FP32 MFMA blocks (complex64 state) against the FP64 oracle.
+
+Bars: one block application within 2e-6 max amplitude error (FP32 storage and
+accumulation, ~1e-7 relative); whole fused circuits at fidelity >= 1 - 1e-6
+against the FP64 oracle state.  Block qubit sets cover the low run of qubits
+(inside the contiguous 7-bit tile run), high qubits and mixtures, so every
+tile-addressing case of the kernel is exercised.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from qpanda_amd import circuits as C
+from qpanda_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+QSETS = {
+    4: [[0, 1, 2, 3], [3, 2, 1, 0], [12, 13, 14, 15], [0, 7, 11, 15], [5, 9, 6, 14], [15, 0, 8, 3]],
+    5: [[0, 1, 2, 3, 4], [11, 12, 13, 14, 15], [0, 6, 7, 13, 2], [15, 9, 1, 4, 10]],
+    2: [[0, 1], [14, 3]],
+    3: [[0, 5, 15], [8, 9, 10]],
+}
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+@pytest.mark.parametrize("split", [False, True])
+def test_block_mfma_vs_oracle(k, split):
+    n = 16
+    for idx, qs in enumerate(QSETS[k]):
+        u = C.haar_unitary(C.Xoshiro256(31 * k + idx), 1 << k)
+        psi = C.random_gaussian_state(n, 7 * k + idx)
+        st = O.State(n)
+        st.set(psi)
+        st.block(qs, u)
+        say("block", k, qs, "split", split)
+        with E.StateVector(n, E.C64) as sv:
+            sv.set(psi)
+            sv.block(qs, u, split=split)
+            got = sv.get()
+        err = np.abs(got - st.psi).max()
+        assert err < 2e-6 * np.abs(st.psi).max() * 64, (qs, err)
+        fid = abs(np.vdot(st.psi, got)) ** 2 / (np.vdot(got, got).real * np.vdot(st.psi, st.psi).real)
+        assert fid > 1 - 1e-10, (qs, fid)
+
+
+def test_split_is_not_less_accurate():
+    n = 16
+    qs = [2, 9, 13, 4, 6]
+    u = C.haar_unitary(C.Xoshiro256(99), 32)
+    psi = C.random_gaussian_state(n, 3)
+    st = O.State(n)
+    st.set(psi)
+    for _ in range(20):
+        st.block(qs, u)
+    errs = {}
+    for split in (False, True):
+        with E.StateVector(n, E.C64) as sv:
+            sv.set(psi)
+            for _ in range(20):
+                sv.block(qs, u, split=split)
+            errs[split] = np.abs(sv.get() - st.psi).max()
+    say("20 blocks, max error without / with split", errs[False], errs[True])
+    assert errs[True] <= errs[False] * 1.1
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+def test_fused_circuit_fidelity(k):
+    n = 16
+    circ = C.random_layers(n, 20, 500 + k, haar_frac=0.1)
+    ref, _ = O.run_circuit(circ)
+    say("fused circuit k", k)
+    with E.StateVector(n, E.C64) as sv:
+        st = sv.run(circ, fuse_k=k, split=True)
+        got = sv.get()
+    assert st.blocks > 0 and st.blocks < circ.ngates
+    fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
+    say("k", k, "blocks", st.blocks, "fidelity", fid)
+    assert fid >= 1 - 1e-6
+    assert np.abs(got - ref.psi).max() < 1e-5
+
+
+def test_fused_matches_plain_c64_and_device_fidelity():
+    n = 18
+    circ = C.random_layers(n, 12, 77, haar_frac=0.1)
+    with E.StateVector(n, E.C128) as p, E.StateVector(n, E.C64) as f:
+        p.run(circ)
+        f.run(circ, fuse_k=4, split=True)
+        fid = p.fidelity(f)
+        ref = p.get()
+        got = f.get()
+    host = abs(np.vdot(ref, got)) ** 2 / (np.vdot(got, got).real * np.vdot(ref, ref).real)
+    assert abs(fid - host) < 1e-12
+    assert fid >= 1 - 1e-6
+
+
+def test_fused_circuit_with_measurements_and_rng_accounting():
+    n = 14
+    b = C._Builder(n)
+    base = C.random_layers(n, 6, 9)
+    for g in base.gates:
+        b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    b.add(C.MEASURE, 3, 0)
+    b.add(C.RESET, 5)
+    for q in range(n):
+        b.add(C.H, q)
+    circ = b.build()
+    out = np.zeros(64, np.int32)
+    with E.StateVector(n, E.C64) as f, E.StateVector(n, E.C128) as p:
+        f.run(circ, fuse_k=4, rng_state=4321, outcomes=out)
+        rng_fused = f.last_rng
+        out_p = np.zeros(64, np.int32)
+        p.run(circ, rng_state=4321, outcomes=out_p)
+        assert rng_fused == p.last_rng          # one draw per logical gate, as the reference
+        assert abs(f.norm2() - 1) < 1e-5
+        if out[0] == out_p[0]:
+            assert f.fidelity(p) >= 1 - 1e-6
+
+
+def test_fused_large_state_sweep_30q():
+    n = 30
+    circ = C.random_layers(n, 2, 1, haar_frac=0.1)
+    say("fused 30q depth 2")
+    with E.StateVector(n, E.C64) as sv:
+        st = sv.run(circ, fuse_k=4, split=True, profile=True)
+        assert abs(sv.norm2() - (2 * 0.707106781186548 ** 2) ** circ.counts().get("H", 0)) < 1e-4
+    c = st.classes()["fused"]
+    say("fused launches", c["launches"], "ms/launch", c["ms"] / c["launches"], "GB/s", c["gbs"])
