@@ -80,6 +80,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel_class, a, headline):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json, scaled by the copy-kernel
+    calibration), when they cover this kernel at this size; else None."""
+    key = None
+    if kernel_class == "fused" and a.fuse_k == 4:
+        key = f"k_fused_mfma<4, true>@{a.n}"
+    elif kernel_class == "dense1" and headline == "plain":
+        key = f"k_pair<double2, 2>@{a.n}"
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+        return t[key]["bytes_per_launch"] if key in t else None
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def cpu_baseline(circ, seconds, stride):
     """QPanda's CPU simulator (reference build) on records 0, stride, 2*stride, ...
     of the benchmark circuit; falls back to the oracle port when the reference
@@ -134,8 +151,8 @@ def main():
     fused_ok = a.path in ("auto", "fused")
     if fused_ok:
         try:
-            with E.StateVector(2, E.C64) as probe:
-                probe.run(C.ghz(2), fuse_k=2)
+            with E.StateVector(12, E.C64) as chk:
+                chk.run(C.ghz(12), fuse_k=4)
         except E.QsvError as e:
             if a.path == "fused":
                 raise
@@ -215,7 +232,7 @@ def main():
         achieved = d["bytes"] / (d["ms"] * 1e6)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(dom, a, headline)
     roof["kernel_class"] = dom
     roof["avg_launch_ms"] = per_launch_ms
     roof["algorithmic_bytes_per_launch"] = d["bytes"] / d["launches"]
