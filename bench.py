@@ -82,7 +82,7 @@ def log(*a):
 
 def measured_traffic(kernel_class, a, headline):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json, scaled by the copy-kernel
+    FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_per_launch.txt, scaled by the copy-kernel
     calibration), when they cover this kernel at this size; else None."""
     key = None
     if kernel_class == "fused" and a.fuse_k == 4:
@@ -90,7 +90,7 @@ def measured_traffic(kernel_class, a, headline):
     elif kernel_class == "dense1" and headline == "plain":
         key = f"k_pair<double2, 2>@{a.n}"
     try:
-        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.txt")) as f:
             t = json.load(f)
         return t[key]["bytes_per_launch"] if key in t else None
     except (OSError, ValueError, KeyError):
