@@ -246,7 +246,9 @@ typedef struct qsv_run_opts {
     int split;          /* fused blocks: 1 = hi+lo split matrix product         */
     int scheduler;      /* 0: greedy fusion, 1: learned cost model              */
     int profile;        /* 1: time every launch with events (per-class stats)   */
-    int reserved[3];
+    int tile;           /* fused blocks: tile column bits 6, 7 or 8 (0 = 7);   */
+                        /* the learned scheduler picks per block instead        */
+    int reserved[2];
 } qsv_run_opts;
 
 /* Kernel classes for per-launch profiling. */
@@ -292,6 +294,40 @@ int qsv_fuse_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int
 int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngates,
                     const double *mats, const qsv_run_opts *opts,
                     int32_t *rng, int *outcomes, qsv_run_stats *stats);
+
+/* ---- learned cost model for the fused path ----------------------------------
+ * A small MLP (QSV_CM_FEATURES -> 32 -> 32 -> 1, ReLU) predicting
+ * log(device ms of one fused block, scaled to 2^30 amplitudes) from the block's
+ * features.  Forward, backward and Adam run as HIP kernels on the device;
+ * training data are the per-block timings the engine records itself
+ * (qsv_timing_log).  The scheduler (opts.scheduler = 1) plans the circuit at
+ * every block size k = 2..5, predicts every block at every tile shape in one
+ * batched forward pass, and keeps the plan with the least predicted time; it
+ * falls back to the greedy plan at opts.fuse_k when no model is attached.
+ * Plans differ only in grouping and tiling, never in the gates applied.
+ * No reference counterpart. */
+#define QSV_CM_FEATURES 12
+typedef struct qsv_costmodel qsv_costmodel;
+int qsv_cm_create(uint64_t seed, qsv_costmodel **out);
+int qsv_cm_destroy(qsv_costmodel *m);
+int qsv_cm_nparams(void);
+int qsv_cm_get_params(qsv_costmodel *m, float *params);
+int qsv_cm_set_params(qsv_costmodel *m, const float *params);
+/* features of one fused block (host computation) */
+int qsv_cm_block_features(int n, int kq, int split, int tile, const int *qubits, float *features);
+/* batched inference: y[i] = model(x[i]) for i < batch (x: batch x QSV_CM_FEATURES) */
+int qsv_cm_forward(qsv_costmodel *m, const float *x, int64_t batch, float *y);
+/* gradient of 0.5 * mean((model(x) - t)^2) with respect to the parameters */
+int qsv_cm_gradients(qsv_costmodel *m, const float *x, const float *t, int64_t batch, float *grad,
+                     double *loss);
+/* full-batch Adam for `epochs` steps; *loss = loss after the last step */
+int qsv_cm_train(qsv_costmodel *m, const float *x, const float *t, int64_t batch, int epochs, float lr,
+                 double *loss);
+/* attach (or detach with NULL) the model used by opts.scheduler = 1 */
+int qsv_set_costmodel(qsv_state *s, qsv_costmodel *m);
+/* per-block records of the last profiled fused run: features (count x
+ * QSV_CM_FEATURES) and measured device ms (count); *count = records available */
+int qsv_timing_log(qsv_state *s, float *features, float *ms, int64_t cap, int64_t *count);
 
 /* ---- Park-Miller "minimal standard" generator (QuantumGates.cpp:46-85) ----- */
 /* Next state from `state` (1..2^31-2); the draw is state / 2147483647.0. */

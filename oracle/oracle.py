@@ -77,6 +77,13 @@ def lib():
         L.orc_run.restype = I
         L.orc_run.argtypes = [P, I, C.c_void_p, C.c_int64, C.c_void_p, I,
                               C.POINTER(C.c_int32), C.c_void_p]
+        PD = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.orc_mlp_nparams.restype = C.c_int
+        L.orc_mlp_forward.argtypes = [PD, PD, C.c_int64, PD]
+        L.orc_mlp_loss.restype = C.c_double
+        L.orc_mlp_loss.argtypes = [PD, PD, PD, C.c_int64]
+        L.orc_mlp_grad.restype = C.c_double
+        L.orc_mlp_grad.argtypes = [PD, PD, PD, C.c_int64, PD]
         L.orc_sample_shots.argtypes = [P, I, C.POINTER(C.c_int), I, C.c_int64,
                                        C.POINTER(C.c_int32), C.c_int64,
                                        np.ctypeslib.ndpointer(dtype=np.uint64), P]
@@ -175,6 +182,30 @@ class State:
         if rc != 0:
             raise ValueError("oracle: bad gate record")
         return rs.value
+
+
+def mlp_forward(params, x):
+    """FP64 forward pass of the cost-model network (oracle/mlp_oracle.c)."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 12)
+    y = np.zeros(x.shape[0], np.float64)
+    lib().orc_mlp_forward(p, x.ravel(), x.shape[0], y)
+    return y
+
+
+def mlp_loss(params, x, t):
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 12)
+    return lib().orc_mlp_loss(p, x.ravel(), np.ascontiguousarray(t, np.float64), x.shape[0])
+
+
+def mlp_grad(params, x, t):
+    """(gradient, loss) of 0.5 * mean((y - t)^2), FP64 chain rule."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 12)
+    g = np.zeros(p.size, np.float64)
+    loss = lib().orc_mlp_grad(p, x.ravel(), np.ascontiguousarray(t, np.float64), x.shape[0], g)
+    return g, loss
 
 
 def pm_next(s: int) -> int:

@@ -184,11 +184,12 @@ int Profiler::begin()
         cls_.push_back(-1);
         bytes_.push_back(0);
         flops_.push_back(0);
+        tags_.push_back(-1);
     }
     return QSV_OK;
 }
 
-int Profiler::end(int cls, double bytes, double flops)
+int Profiler::end(int cls, double bytes, double flops, int64_t tag)
 {
     if (!on_) return QSV_OK;
     hipEvent_t e;
@@ -198,6 +199,7 @@ int Profiler::end(int cls, double bytes, double flops)
     cls_.push_back(cls);
     bytes_.push_back(bytes);
     flops_.push_back(flops);
+    tags_.push_back(tag);
     return QSV_OK;
 }
 
@@ -213,6 +215,10 @@ int Profiler::collect(qsv_run_stats *st)
         st->class_ms[cls_[i]] += ms;
         st->class_bytes[cls_[i]] += bytes_[i];
         st->class_flops[cls_[i]] += flops_[i];
+        if (tags_[i] >= 0) {
+            if ((size_t)tags_[i] >= tag_ms_.size()) tag_ms_.resize(tags_[i] + 1, 0.0f);
+            tag_ms_[tags_[i]] = ms;
+        }
     }
     return QSV_OK;
 }
@@ -276,6 +282,10 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
         stats->device_ms = ms;
         stats->model_ms_pred = pred_ms;
         if (!rc) rc = prof.collect(stats);
+    }
+    if (!rc && prof.on() && o.fuse_k >= 2) {
+        s->tlog_ms = prof.tag_ms();
+        s->tlog_ms.resize(s->tlog_feats.size() / QSV_CM_FEATURES, 0.0f);
     }
     return rc;
 }

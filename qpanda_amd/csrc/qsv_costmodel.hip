@@ -1,13 +1,419 @@
-// qsv_costmodel.hip -- learned scheduler (placeholder: greedy at the requested k).
+// qsv_costmodel.hip -- learned cost model for fused blocks and the scheduler
+// that uses it.
+//
+// Model: MLP F=12 -> 32 -> 32 -> 1 with ReLU, predicting
+//   y = log(ms of one fused-block launch, scaled to 2^30 amplitudes).
+// Parameters (1505 floats): W1[32][12], b1[32], W2[32][32], b2[32], W3[32], b3.
+// Kernels (all deterministic):
+//   k_mlp_forward   one thread per sample, activations kept for backward;
+//   k_mlp_backward  one thread per sample, per-sample gradient of
+//                   0.5 * mean((y - t)^2) into a [batch][P] buffer;
+//   k_grad_reduce   one thread per parameter, sums the batch in index order;
+//   k_adam          Adam update.
+// The scheduler plans the circuit at k = 2..5, evaluates every block at every
+// tile shape in one batched forward pass, and keeps the cheapest plan; the
+// greedy planner at opts.fuse_k is the fallback when no model is attached.
+#include "qsv_fusion.h"
 #include "qsv_sched.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+struct qsv_costmodel {
+    float *params = nullptr;     // device, kParams
+    float *m = nullptr, *v = nullptr, *grad = nullptr;
+    hipStream_t stream = nullptr;
+    int64_t step = 0;
+};
 
 namespace qsv {
 
-int plan_learned(qsv_state *, const qsv_gate *gates, int64_t ng, const double *mats, const qsv_run_opts &o,
+namespace {
+
+constexpr int F = QSV_CM_FEATURES, H = 32;
+constexpr int OW1 = 0, OB1 = OW1 + H * F, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
+constexpr int kParams = OB3 + 1;
+
+__global__ void k_mlp_forward(const float *__restrict__ p, const float *__restrict__ x, int64_t batch,
+                              float *__restrict__ y, float *__restrict__ act)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    float h1[H], h2[H];
+    const float *xb = x + b * F;
+    for (int j = 0; j < H; j++) {
+        float s = p[OB1 + j];
+        for (int i = 0; i < F; i++) s += p[OW1 + j * F + i] * xb[i];
+        h1[j] = s > 0 ? s : 0;
+    }
+    float out = p[OB3];
+    for (int j = 0; j < H; j++) {
+        float s = p[OB2 + j];
+        for (int i = 0; i < H; i++) s += p[OW2 + j * H + i] * h1[i];
+        h2[j] = s > 0 ? s : 0;
+        out += p[OW3 + j] * h2[j];
+    }
+    y[b] = out;
+    if (act) {
+        for (int j = 0; j < H; j++) {
+            act[b * 2 * H + j] = h1[j];
+            act[b * 2 * H + H + j] = h2[j];
+        }
+    }
+}
+
+__global__ void k_mlp_backward(const float *__restrict__ p, const float *__restrict__ x,
+                               const float *__restrict__ act, const float *__restrict__ y,
+                               const float *__restrict__ t, int64_t batch, float *__restrict__ g)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    const float *xb = x + b * F, *h1 = act + b * 2 * H, *h2 = h1 + H;
+    float *gb = g + b * kParams;
+    const float dy = (y[b] - t[b]) / (float)batch;
+    gb[OB3] = dy;
+    float dh2[H];
+    for (int j = 0; j < H; j++) {
+        gb[OW3 + j] = dy * h2[j];
+        dh2[j] = h2[j] > 0 ? dy * p[OW3 + j] : 0.0f;
+        gb[OB2 + j] = dh2[j];
+    }
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < H; i++) gb[OW2 + j * H + i] = dh2[j] * h1[i];
+    for (int i = 0; i < H; i++) {
+        float s = 0;
+        for (int j = 0; j < H; j++) s += p[OW2 + j * H + i] * dh2[j];
+        const float dh1 = h1[i] > 0 ? s : 0.0f;
+        gb[OB1 + i] = dh1;
+        for (int k = 0; k < F; k++) gb[OW1 + i * F + k] = dh1 * xb[k];
+    }
+}
+
+__global__ void k_grad_reduce(const float *__restrict__ g, int64_t batch, float *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kParams) return;
+    float s = 0;
+    for (int64_t b = 0; b < batch; b++) s += g[b * kParams + p];
+    out[p] = s;
+}
+
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kParams) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    m[i] = b1 * m[i] + (1 - b1) * g[i];
+    v[i] = b2 * v[i] + (1 - b2) * g[i] * g[i];
+    p[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+}
+
+uint64_t splitmix(uint64_t &x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Loss of the current parameters on (x, t), plus the device buffers it used.
+struct Batch {
+    float *x = nullptr, *t = nullptr, *y = nullptr, *act = nullptr, *g = nullptr;
+    int64_t n = 0;
+    ~Batch()
+    {
+        for (float *p : {x, t, y, act, g})
+            if (p) (void)hipFree(p);
+    }
+};
+
+int upload_batch(qsv_costmodel *m, const float *x, const float *t, int64_t batch, bool need_grad, Batch &B)
+{
+    B.n = batch;
+    if (hipMalloc((void **)&B.x, sizeof(float) * F * batch) != hipSuccess ||
+        hipMalloc((void **)&B.y, sizeof(float) * batch) != hipSuccess ||
+        hipMalloc((void **)&B.act, sizeof(float) * 2 * H * batch) != hipSuccess)
+        return fail(QSV_ERR_ALLOC, "cost-model batch");
+    QSV_HIP(hipMemcpyAsync(B.x, x, sizeof(float) * F * batch, hipMemcpyHostToDevice, m->stream));
+    if (t) {
+        if (hipMalloc((void **)&B.t, sizeof(float) * batch) != hipSuccess) return fail(QSV_ERR_ALLOC, "targets");
+        QSV_HIP(hipMemcpyAsync(B.t, t, sizeof(float) * batch, hipMemcpyHostToDevice, m->stream));
+    }
+    if (need_grad && hipMalloc((void **)&B.g, sizeof(float) * kParams * batch) != hipSuccess)
+        return fail(QSV_ERR_ALLOC, "per-sample gradients");
+    return QSV_OK;
+}
+
+int forward(qsv_costmodel *m, Batch &B)
+{
+    const unsigned grid = (unsigned)((B.n + 127) / 128);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(128), 0, m->stream, m->params, B.x, B.n, B.y, B.act);
+    QSV_HIP(hipGetLastError());
+    return QSV_OK;
+}
+
+int backward(qsv_costmodel *m, Batch &B)
+{
+    const unsigned grid = (unsigned)((B.n + 127) / 128);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(grid), dim3(128), 0, m->stream, m->params, B.x, B.act, B.y, B.t, B.n, B.g);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, B.g, B.n, m->grad);
+    QSV_HIP(hipGetLastError());
+    return QSV_OK;
+}
+
+int batch_loss(qsv_costmodel *m, Batch &B, const float *t_host, double *loss)
+{
+    std::vector<float> y(B.n);
+    QSV_HIP(hipMemcpyAsync(y.data(), B.y, sizeof(float) * B.n, hipMemcpyDeviceToHost, m->stream));
+    QSV_HIP(hipStreamSynchronize(m->stream));
+    double s = 0;
+    for (int64_t i = 0; i < B.n; i++) s += 0.5 * ((double)y[i] - t_host[i]) * ((double)y[i] - t_host[i]);
+    *loss = s / (double)B.n;
+    return QSV_OK;
+}
+
+}  // namespace
+
+// Widened qubit list of a block (the lowest unused qubits fill it to kq).
+static std::vector<int> widened_qubits(const std::vector<int> &qs, int kq)
+{
+    std::vector<int> out = qs;
+    uint64_t used = 0;
+    for (int q : qs) used |= uint64_t(1) << q;
+    for (int b = 0; (int)out.size() < kq; b++)
+        if (!(used >> b & 1)) out.push_back(b);
+    return out;
+}
+
+int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *mats, const qsv_run_opts &o,
                  std::vector<PlanItem> &plan, SchedChoice *choice)
 {
-    choice->fuse_k = o.fuse_k;
-    return plan_greedy(gates, ng, mats, o.fuse_k, o.exact, plan);
+    if (!s->cm || s->n < 4 + 6) {
+        choice->fuse_k = o.fuse_k;
+        return plan_greedy(gates, ng, mats, o.fuse_k, o.exact, plan);
+    }
+    const int tcs[3] = {6, 7, 8};
+    double best_total = std::numeric_limits<double>::infinity();
+    int best_k = o.fuse_k;
+    std::vector<PlanItem> best;
+    for (int k = 2; k <= 5; k++) {
+        std::vector<PlanItem> cand;
+        int rc = plan_greedy(gates, ng, mats, k, o.exact, cand);
+        if (rc) return rc;
+        // features of every block at every tile shape that fits
+        std::vector<float> feats;
+        std::vector<std::pair<size_t, int>> who;      // (block, tc)
+        for (size_t i = 0; i < cand.size(); i++) {
+            if (cand[i].barrier >= 0) continue;
+            const int kq = std::max(4, (int)cand[i].qubits.size());
+            const std::vector<int> wq = widened_qubits(cand[i].qubits, kq);
+            for (int tc : tcs) {
+                if (s->n < kq + tc) continue;
+                float f[F];
+                qsv_cm_block_features(s->n, kq, o.split, tc, wq.data(), f);
+                feats.insert(feats.end(), f, f + F);
+                who.emplace_back(i, tc);
+            }
+        }
+        std::vector<float> y(who.size());
+        if (!who.empty()) {
+            rc = qsv_cm_forward(s->cm, feats.data(), (int64_t)who.size(), y.data());
+            if (rc) return rc;
+        }
+        const double scale = std::ldexp(1.0, s->n - 30);
+        std::vector<double> best_ms(cand.size(), std::numeric_limits<double>::infinity());
+        for (size_t j = 0; j < who.size(); j++) {
+            const double ms = std::exp((double)y[j]) * scale;
+            PlanItem &it = cand[who[j].first];
+            if (ms < best_ms[who[j].first]) {
+                best_ms[who[j].first] = ms;
+                it.tile = who[j].second;
+                it.pred_ms = ms;
+            }
+        }
+        double total = 0;
+        for (size_t i = 0; i < cand.size(); i++)
+            if (cand[i].barrier < 0) total += std::isfinite(best_ms[i]) ? best_ms[i] : 0;
+        if (total < best_total) {
+            best_total = total;
+            best_k = k;
+            best.swap(cand);
+        }
+    }
+    plan.swap(best);
+    choice->fuse_k = best_k;
+    choice->pred_ms = best_total;
+    return QSV_OK;
 }
 
 }  // namespace qsv
+
+using namespace qsv;
+
+extern "C" {
+
+int qsv_cm_nparams(void) { return kParams; }
+
+int qsv_cm_block_features(int n, int kq, int split, int tile, const int *qubits, float *f)
+{
+    if (!qubits || !f || n < 1 || kq < 1 || kq > 5) return fail(QSV_ERR_ARG, "bad feature arguments");
+    int lo = 0, top = 0, above = 0, mn = 64, mx = -1;
+    for (int j = 0; j < kq; j++) {
+        const int q = qubits[j];
+        lo += q < tile;
+        top += q >= n - 3;
+        above += q >= tile && q < tile + 6;
+        mn = std::min(mn, q);
+        mx = std::max(mx, q);
+    }
+    f[0] = n / 34.0f;
+    f[1] = kq == 4;
+    f[2] = kq == 5;
+    f[3] = split ? 1.0f : 0.0f;
+    f[4] = tile == 6;
+    f[5] = tile == 7;
+    f[6] = tile == 8;
+    f[7] = (float)lo / kq;
+    f[8] = (float)mn / n;
+    f[9] = (float)mx / n;
+    f[10] = (float)top / kq;
+    f[11] = (float)above / kq;
+    return QSV_OK;
+}
+
+int qsv_cm_create(uint64_t seed, qsv_costmodel **out)
+{
+    if (!out) return fail(QSV_ERR_ARG, "null");
+    *out = nullptr;
+    qsv_costmodel *m = new qsv_costmodel();
+    bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc((void **)&m->params, sizeof(float) * kParams) == hipSuccess &&
+              hipMalloc((void **)&m->m, sizeof(float) * kParams) == hipSuccess &&
+              hipMalloc((void **)&m->v, sizeof(float) * kParams) == hipSuccess &&
+              hipMalloc((void **)&m->grad, sizeof(float) * kParams) == hipSuccess;
+    if (!ok) {
+        qsv_cm_destroy(m);
+        return fail(QSV_ERR_ALLOC, "cost model allocation");
+    }
+    // He-uniform weights from a seeded SplitMix64 stream, zero biases
+    std::vector<float> p(kParams, 0.0f);
+    uint64_t st = seed;
+    auto uni = [&]() { return (double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0); };
+    auto fill = [&](int off, int count, int fan_in) {
+        const double a = std::sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; i++) p[off + i] = (float)((2 * uni() - 1) * a);
+    };
+    fill(OW1, H * F, F);
+    fill(OW2, H * H, H);
+    fill(OW3, H, H);
+    int rc = qsv_cm_set_params(m, p.data());
+    if (rc) {
+        qsv_cm_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return QSV_OK;
+}
+
+int qsv_cm_destroy(qsv_costmodel *m)
+{
+    if (!m) return QSV_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (float *p : {m->params, m->m, m->v, m->grad})
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return QSV_OK;
+}
+
+int qsv_cm_get_params(qsv_costmodel *m, float *params)
+{
+    if (!m || !params) return fail(QSV_ERR_ARG, "null");
+    QSV_HIP(hipMemcpyAsync(params, m->params, sizeof(float) * kParams, hipMemcpyDeviceToHost, m->stream));
+    QSV_HIP(hipStreamSynchronize(m->stream));
+    return QSV_OK;
+}
+
+int qsv_cm_set_params(qsv_costmodel *m, const float *params)
+{
+    if (!m || !params) return fail(QSV_ERR_ARG, "null");
+    QSV_HIP(hipMemcpyAsync(m->params, params, sizeof(float) * kParams, hipMemcpyHostToDevice, m->stream));
+    QSV_HIP(hipMemsetAsync(m->m, 0, sizeof(float) * kParams, m->stream));
+    QSV_HIP(hipMemsetAsync(m->v, 0, sizeof(float) * kParams, m->stream));
+    QSV_HIP(hipStreamSynchronize(m->stream));
+    m->step = 0;
+    return QSV_OK;
+}
+
+int qsv_cm_forward(qsv_costmodel *m, const float *x, int64_t batch, float *y)
+{
+    if (!m || !x || !y || batch < 0) return fail(QSV_ERR_ARG, "bad argument");
+    if (batch == 0) return QSV_OK;
+    Batch B;
+    int rc = upload_batch(m, x, nullptr, batch, false, B);
+    if (!rc) rc = forward(m, B);
+    if (rc) return rc;
+    QSV_HIP(hipMemcpyAsync(y, B.y, sizeof(float) * batch, hipMemcpyDeviceToHost, m->stream));
+    QSV_HIP(hipStreamSynchronize(m->stream));
+    return QSV_OK;
+}
+
+int qsv_cm_gradients(qsv_costmodel *m, const float *x, const float *t, int64_t batch, float *grad, double *loss)
+{
+    if (!m || !x || !t || !grad || batch < 1) return fail(QSV_ERR_ARG, "bad argument");
+    Batch B;
+    int rc = upload_batch(m, x, t, batch, true, B);
+    if (!rc) rc = forward(m, B);
+    if (!rc) rc = backward(m, B);
+    if (rc) return rc;
+    QSV_HIP(hipMemcpyAsync(grad, m->grad, sizeof(float) * kParams, hipMemcpyDeviceToHost, m->stream));
+    QSV_HIP(hipStreamSynchronize(m->stream));
+    if (loss) return batch_loss(m, B, t, loss);
+    return QSV_OK;
+}
+
+int qsv_cm_train(qsv_costmodel *m, const float *x, const float *t, int64_t batch, int epochs, float lr,
+                 double *loss)
+{
+    if (!m || !x || !t || batch < 1 || epochs < 0) return fail(QSV_ERR_ARG, "bad argument");
+    Batch B;
+    int rc = upload_batch(m, x, t, batch, true, B);
+    if (rc) return rc;
+    for (int e = 0; e < epochs && !rc; e++) {
+        rc = forward(m, B);
+        if (!rc) rc = backward(m, B);
+        if (rc) break;
+        m->step++;
+        const float c1 = 1.0f - std::pow(0.9f, (float)m->step), c2 = 1.0f - std::pow(0.999f, (float)m->step);
+        hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->params, m->grad, m->m,
+                           m->v, lr, c1, c2);
+        hipError_t er = hipGetLastError();
+        if (er != hipSuccess) rc = hip_fail(er, "k_adam");
+    }
+    if (!rc) rc = forward(m, B);
+    if (!rc && loss) rc = batch_loss(m, B, t, loss);
+    return rc;
+}
+
+int qsv_set_costmodel(qsv_state *s, qsv_costmodel *m)
+{
+    if (!s) return fail(QSV_ERR_ARG, "null state");
+    s->cm = m;
+    return QSV_OK;
+}
+
+int qsv_timing_log(qsv_state *s, float *features, float *ms, int64_t cap, int64_t *count)
+{
+    if (!s || !count) return fail(QSV_ERR_ARG, "null");
+    const int64_t n = (int64_t)s->tlog_ms.size();
+    *count = n;
+    const int64_t k = std::min(n, cap);
+    if (features && k > 0) std::memcpy(features, s->tlog_feats.data(), sizeof(float) * F * k);
+    if (ms && k > 0) std::memcpy(ms, s->tlog_ms.data(), sizeof(float) * k);
+    return QSV_OK;
+}
+
+}  // extern "C"

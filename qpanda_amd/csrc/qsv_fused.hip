@@ -35,19 +35,21 @@ namespace qsv {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 struct FusedArgs {
-    uint64_t off_high[32];   // address offset of the tile bits above the 7-bit run
+    uint64_t off_high[32];   // address offset of the tile bits above the low run
     InsertSet tile_ins;      // L positions, to expand a tile index into its base
     int erow[32];            // compact tile position of row (amplitude) r
-    int col_rank[7];         // rank in L of the i-th of the 7 free tile bits
+    int col_rank[8];         // rank in L of the i-th free tile (column) bit
 };
 
-template <int KQ, bool SPLIT>
-__global__ __launch_bounds__(256) void k_fused_mfma(float2 *__restrict__ a, uint64_t ntiles, FusedArgs fa,
-                                                     const float *__restrict__ mops)
+// KQ: block qubits (4, 5).  TC: column bits per tile (6, 7, 8 -> 2, 4, 8 waves).
+template <int KQ, int TC, bool SPLIT>
+__global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, uint64_t ntiles, FusedArgs fa,
+                                                         const float *__restrict__ mops)
 {
     constexpr int D = 1 << KQ;               // amplitudes per group
-    constexpr int TB = KQ + 7;               // tile bits
-    constexpr int TSIZE = 1 << TB;           // amplitudes per tile
+    constexpr int THREADS = 2 << TC;         // one wave per 32 columns
+    constexpr int TSIZE = 1 << (KQ + TC);    // amplitudes per tile
+    constexpr int RUN = 1 << TC;             // contiguous low run (qubits 0..TC-1)
     constexpr int HALVES = (2 * D) / 32;     // 32-row output halves: 1 (k=4) or 2 (k=5)
     constexpr int KSTEPS = D;                // K = 2D real rows, 2 per step
     __shared__ float planes[2][TSIZE];     // [0] = re, [1] = im
@@ -72,20 +74,20 @@ __global__ __launch_bounds__(256) void k_fused_mfma(float2 *__restrict__ a, uint
 
     // load the tile: element pairs (e, e+1) are adjacent in memory (bit 0 is in L)
 #pragma unroll
-    for (int i = 0; i < TSIZE / 512; i++) {
-        const int e = 2 * (tid + 256 * i);
-        const uint64_t addr = base + (uint64_t)(e & 127) + fa.off_high[e >> 7];
+    for (int i = 0; i < TSIZE / (2 * THREADS); i++) {
+        const int e = 2 * (tid + THREADS * i);
+        const uint64_t addr = base + (uint64_t)(e & (RUN - 1)) + fa.off_high[e >> TC];
         const float4 v = *reinterpret_cast<const float4 *>(a + addr);
         pre[e] = v.x; pim[e] = v.y;
         pre[e + 1] = v.z; pim[e + 1] = v.w;
     }
     __syncthreads();
 
-    // this lane's column within the wave's 32 and its compact position bits
+    // this lane's column and its compact position bits
     const int col = wave * 32 + (lane & 31);
     int ecol = 0;
 #pragma unroll
-    for (int i = 0; i < 7; i++)
+    for (int i = 0; i < TC; i++)
         if (col >> i & 1) ecol |= 1 << fa.col_rank[i];
     const float *const bplane = planes[lane >> 5];   // lanes 32-63 feed the im rows
 
@@ -118,9 +120,9 @@ __global__ __launch_bounds__(256) void k_fused_mfma(float2 *__restrict__ a, uint
         }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < TSIZE / 512; i++) {
-        const int e = 2 * (tid + 256 * i);
-        const uint64_t addr = base + (uint64_t)(e & 127) + fa.off_high[e >> 7];
+    for (int i = 0; i < TSIZE / (2 * THREADS); i++) {
+        const int e = 2 * (tid + THREADS * i);
+        const uint64_t addr = base + (uint64_t)(e & (RUN - 1)) + fa.off_high[e >> TC];
         float4 v;
         v.x = pre[e]; v.y = pim[e];
         v.z = pre[e + 1]; v.w = pim[e + 1];
@@ -156,31 +158,42 @@ size_t fused_operand_floats(int kq, int split)
     return (size_t)(split ? 2 : 1) * ((2 * d) / 32) * d * 64;
 }
 
-// Launch one fused block (kq = 4 or 5, qubits[0..kq-1] distinct) with
-// operands already on the device.
-int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split)
+template <int KQ, int TC>
+static void launch_kq_tc(qsv_state *s, uint64_t ntiles, const FusedArgs &fa, const float *mops, int split)
 {
+    float2 *a = (float2 *)s->amps;
+    const dim3 g((unsigned)ntiles), t(2 << TC);
+    if (split) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, true>), g, t, 0, s->stream, a, ntiles, fa, mops);
+    else hipLaunchKernelGGL((k_fused_mfma<KQ, TC, false>), g, t, 0, s->stream, a, ntiles, fa, mops);
+}
+
+// Launch one fused block (kq = 4 or 5, qubits[0..kq-1] distinct) with
+// operands already on the device.  tc = tile column bits (6, 7, 8; 0 = 7).
+int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split, int tc)
+{
+    if (tc == 0) tc = kDefaultTileCols;
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused MFMA blocks need a c64 state");
     if (kq != 4 && kq != 5) return fail(QSV_ERR_ARG, "fused block size must be 4 or 5");
-    if (s->n < kq + 7) return fail(QSV_ERR_ARG, "state too small for a fused tile");
+    if (tc < 6 || tc > 8) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
+    if (s->n < kq + tc) return fail(QSV_ERR_ARG, "state too small for a fused tile");
     FusedArgs fa{};
     uint64_t qmask = 0;
     for (int j = 0; j < kq; j++) qmask |= uint64_t(1) << qubits[j];
-    // the 7 lowest free bits
-    int free_bits[7], nf = 0;
-    for (int b = 0; b < s->n && nf < 7; b++)
+    // the tc lowest free bits are the tile's columns
+    int free_bits[8], nf = 0;
+    for (int b = 0; b < s->n && nf < tc; b++)
         if (!(qmask >> b & 1)) free_bits[nf++] = b;
     uint64_t lmask = qmask;
-    for (int i = 0; i < 7; i++) lmask |= uint64_t(1) << free_bits[i];
+    for (int i = 0; i < tc; i++) lmask |= uint64_t(1) << free_bits[i];
     fa.tile_ins = insert_set_from_mask(lmask);
-    int lpos[12];
+    int lpos[16];
     for (int i = 0; i < fa.tile_ins.n; i++) lpos[i] = fa.tile_ins.pos[i];
     auto rank = [&](int bit) {
-        for (int i = 0; i < kq + 7; i++)
+        for (int i = 0; i < kq + tc; i++)
             if (lpos[i] == bit) return i;
         return -1;
     };
-    for (int i = 0; i < 7; i++) fa.col_rank[i] = rank(free_bits[i]);
+    for (int i = 0; i < tc; i++) fa.col_rank[i] = rank(free_bits[i]);
     const int d = 1 << kq;
     for (int r = 0; r < d; r++) {
         int e = 0;
@@ -188,21 +201,22 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
             if (r >> j & 1) e |= 1 << rank(qubits[j]);
         fa.erow[r] = e;
     }
-    // L positions 0..6 are qubits 0..6; ranks >= 7 map the high tile bits
+    // L positions 0..tc-1 are qubits 0..tc-1; ranks >= tc map the high tile bits
     for (int hbits = 0; hbits < d; hbits++) {
         uint64_t o = 0;
         for (int j = 0; j < kq; j++)
-            if (hbits >> j & 1) o |= uint64_t(1) << lpos[7 + j];
+            if (hbits >> j & 1) o |= uint64_t(1) << lpos[tc + j];
         fa.off_high[hbits] = o;
     }
-    const uint64_t ntiles = s->dim >> (kq + 7);
-    float2 *a = (float2 *)s->amps;
+    const uint64_t ntiles = s->dim >> (kq + tc);
     if (kq == 4) {
-        if (split) hipLaunchKernelGGL((k_fused_mfma<4, true>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
-        else hipLaunchKernelGGL((k_fused_mfma<4, false>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
+        if (tc == 6) launch_kq_tc<4, 6>(s, ntiles, fa, mops, split);
+        else if (tc == 7) launch_kq_tc<4, 7>(s, ntiles, fa, mops, split);
+        else launch_kq_tc<4, 8>(s, ntiles, fa, mops, split);
     } else {
-        if (split) hipLaunchKernelGGL((k_fused_mfma<5, true>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
-        else hipLaunchKernelGGL((k_fused_mfma<5, false>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, fa, mops);
+        if (tc == 6) launch_kq_tc<5, 6>(s, ntiles, fa, mops, split);
+        else if (tc == 7) launch_kq_tc<5, 7>(s, ntiles, fa, mops, split);
+        else launch_kq_tc<5, 8>(s, ntiles, fa, mops, split);
     }
     s->launches++;
     QSV_HIP(hipGetLastError());
@@ -216,7 +230,7 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
 int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int split)
 {
     int kq = k < 4 ? 4 : k;
-    if (s->n < kq + 7) return launch_block(s, qubits, k, u, 0);
+    if (s->n < kq + kDefaultTileCols) return launch_block(s, qubits, k, u, 0);
     std::vector<int> qs(qubits, qubits + k);
     std::vector<double> w;
     const double *uu = u;
@@ -242,7 +256,7 @@ int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, i
     const size_t bytes = ops.size() * sizeof(float);
     if (bytes > sizeof(double) * (kRedSlots - 8)) return fail(QSV_ERR_ARG, "operand block too large");
     QSV_HIP(hipMemcpyAsync(s->red, ops.data(), bytes, hipMemcpyHostToDevice, s->stream));
-    int rc = launch_fused_dev(s, qs.data(), kq, (const float *)s->red, split);
+    int rc = launch_fused_dev(s, qs.data(), kq, (const float *)s->red, split, 0);
     if (!rc) QSV_HIP(hipStreamSynchronize(s->stream));   // ops is a host temporary
     return rc;
 }

@@ -337,17 +337,22 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
         choice.fuse_k = o.fuse_k;
     }
     if (rc) return rc;
+    if (o.tile != 0 && (o.tile < 6 || o.tile > 8)) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
     const int kq_min = 4;
-    const bool tiles_fit = s->n >= kq_min + 7;
-    // operands for every block in one upload
+    // operands for every block in one upload; blocks whose tile does not fit
+    // the state run on the FP64 block kernel instead
     std::vector<size_t> op_off(plan.size(), 0);
+    std::vector<char> mfma(plan.size(), 0);
     std::vector<float> ops;
     for (size_t i = 0; i < plan.size(); i++) {
         PlanItem &it = plan[i];
         if (it.barrier >= 0) continue;
-        if (!tiles_fit) continue;
+        if (o.scheduler != 1) it.tile = o.tile;
+        const int tc = it.tile ? it.tile : kDefaultTileCols;
+        const int kq = std::max(kq_min, (int)it.qubits.size());
+        if (s->n < kq + tc) continue;
         widen_to(it, kq_min);
-        const int kq = (int)it.qubits.size();
+        mfma[i] = 1;
         op_off[i] = ops.size();
         ops.resize(ops.size() + fused_operand_floats(kq, o.split));
         fused_operands(kq, it.u.data(), o.split, ops.data() + op_off[i]);
@@ -367,6 +372,10 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
     double pred = 0;
     int64_t nblocks = 0;
     rc = QSV_OK;
+    if (prof->on()) {
+        s->tlog_feats.clear();
+        s->tlog_ms.clear();
+    }
     for (size_t i = 0; i < plan.size() && !rc; i++) {
         const PlanItem &it = plan[i];
         if (it.barrier >= 0) {
@@ -391,9 +400,17 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
         if (rng) *rng = qsv_pm_jump(*rng, (uint64_t)it.ngates);
         const int kq = (int)it.qubits.size();
         if ((rc = prof->begin())) break;
-        if (tiles_fit) {
-            rc = launch_fused_dev(s, it.qubits.data(), kq, dops + op_off[i], o.split);
-            if (!rc) rc = prof->end(QSV_K_FUSED, 2 * 8 * N, 8.0 * N * (1 << kq) * (o.split ? 2 : 1));
+        if (mfma[i]) {
+            const int tc = it.tile ? it.tile : kDefaultTileCols;
+            rc = launch_fused_dev(s, it.qubits.data(), kq, dops + op_off[i], o.split, tc);
+            int64_t tag = -1;
+            if (!rc && prof->on()) {
+                float f[QSV_CM_FEATURES];
+                qsv_cm_block_features(s->n, kq, o.split, tc, it.qubits.data(), f);
+                tag = (int64_t)(s->tlog_feats.size() / QSV_CM_FEATURES);
+                s->tlog_feats.insert(s->tlog_feats.end(), f, f + QSV_CM_FEATURES);
+            }
+            if (!rc) rc = prof->end(QSV_K_FUSED, 2 * 8 * N, 8.0 * N * (1 << kq) * (o.split ? 2 : 1), tag);
         } else {
             rc = launch_block(s, it.qubits.data(), kq, it.u.data(), 0);
             if (!rc) rc = prof->end(QSV_K_BLOCK64, 2 * 8 * N, 8.0 * N * (1 << kq));

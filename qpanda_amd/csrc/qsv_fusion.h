@@ -16,9 +16,11 @@ public:
     Profiler(qsv_state *s, bool on);
     ~Profiler();
     int begin();
-    int end(int cls, double bytes, double flops);
+    // tag >= 0 records this launch's time under that index (collect -> tag_ms)
+    int end(int cls, double bytes, double flops, int64_t tag = -1);
     int collect(qsv_run_stats *st);
     bool on() const { return on_; }
+    const std::vector<float> &tag_ms() const { return tag_ms_; }
 
 private:
     qsv_state *s_;
@@ -26,6 +28,8 @@ private:
     std::vector<hipEvent_t> events_;
     std::vector<int> cls_;
     std::vector<double> bytes_, flops_;
+    std::vector<int64_t> tags_;
+    std::vector<float> tag_ms_;
 };
 
 // Fuse `gates` into blocks of at most opts.fuse_k qubits and apply them.

@@ -72,6 +72,9 @@ struct qsv_state {
     int device = 0;
     int64_t launches = 0;
     int64_t tie_resolutions = 0;
+    struct qsv_costmodel *cm = nullptr;   // attached cost model (not owned)
+    std::vector<float> tlog_feats;        // per-block features of the last profiled fused run
+    std::vector<float> tlog_ms;           // and their measured device ms
 };
 
 namespace qsv {

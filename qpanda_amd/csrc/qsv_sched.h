@@ -37,6 +37,7 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
 // Fused MFMA kernel helpers (qsv_fused.hip).
 void fused_operands(int kq, const double *u, int split, float *out);
 size_t fused_operand_floats(int kq, int split);
-int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split);
+constexpr int kDefaultTileCols = 7;
+int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split, int tc);
 
 }  // namespace qsv

@@ -26,7 +26,13 @@ EXPORTED = [
     "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
     "qsv_sample_shots", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
     "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan",
+    "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
+    "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
+    "qsv_timing_log",
 ]
+
+CM_FEATURES = 12
+DEFAULT_COSTMODEL = os.path.join(HERE, "costmodel_params.f32")
 
 
 class QsvError(RuntimeError):
@@ -35,7 +41,8 @@ class QsvError(RuntimeError):
 
 class RunOpts(C.Structure):
     _fields_ = [("exact", C.c_int), ("fuse_k", C.c_int), ("split", C.c_int),
-                ("scheduler", C.c_int), ("profile", C.c_int), ("reserved", C.c_int * 3)]
+                ("scheduler", C.c_int), ("profile", C.c_int), ("tile", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 class RunStats(C.Structure):
@@ -115,6 +122,17 @@ def lib():
             "qsv_device_count": ([C.POINTER(I)], I),
             "qsv_fuse_plan": ([vp, C.c_int64, vp, I, I, C.c_int64, vp, vp, vp, vp,
                                C.POINTER(C.c_int64)], I),
+            "qsv_cm_create": ([U64, C.POINTER(vp)], I),
+            "qsv_cm_destroy": ([vp], I),
+            "qsv_cm_nparams": ([], I),
+            "qsv_cm_get_params": ([vp, vp], I),
+            "qsv_cm_set_params": ([vp, vp], I),
+            "qsv_cm_block_features": ([I, I, I, I, C.POINTER(I), vp], I),
+            "qsv_cm_forward": ([vp, vp, C.c_int64, vp], I),
+            "qsv_cm_gradients": ([vp, vp, vp, C.c_int64, vp, C.POINTER(D)], I),
+            "qsv_cm_train": ([vp, vp, vp, C.c_int64, I, C.c_float, C.POINTER(D)], I),
+            "qsv_set_costmodel": ([vp, vp], I),
+            "qsv_timing_log": ([vp, vp, vp, C.c_int64, C.POINTER(C.c_int64)], I),
             "qsv_copy_state": ([vp, vp], I),
             "qsv_inner": ([vp, vp, C.POINTER(D), C.POINTER(D)], I),
         }
@@ -187,6 +205,19 @@ class StateVector:
 
     def sync(self):
         _check(lib().qsv_sync(self.h))
+
+    def attach_costmodel(self, model: "CostModel | None"):
+        self._cm = model                      # keep the model alive while attached
+        _check(lib().qsv_set_costmodel(self.h, model.h if model is not None else None))
+
+    def timing_log(self):
+        """(features [count x 12], measured ms [count]) of the last profiled fused run."""
+        cnt = C.c_int64()
+        _check(lib().qsv_timing_log(self.h, None, None, 0, C.byref(cnt)))
+        f = np.zeros((cnt.value, CM_FEATURES), np.float32)
+        ms = np.zeros(cnt.value, np.float32)
+        _check(lib().qsv_timing_log(self.h, _fptr(f), _fptr(ms), cnt.value, C.byref(cnt)))
+        return f, ms
 
     def copy_from(self, other: "StateVector"):
         _check(lib().qsv_copy_state(self.h, other.h))
@@ -278,10 +309,10 @@ class StateVector:
         return out, rs.value, ties.value
 
     def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=True, scheduler=0,
-            rng_state=None, outcomes=None, profile=False) -> RunStats:
+            rng_state=None, outcomes=None, profile=False, tile=0) -> RunStats:
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
-        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile))
+        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile))
         st = RunStats()
         rs = C.c_int32(rng_state if rng_state is not None else 1)
         self._last_rng = None
@@ -296,6 +327,81 @@ class StateVector:
     @property
     def last_rng(self):
         return self._last_rng
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def block_features(n, kq, split, tile, qubits) -> np.ndarray:
+    f = np.zeros(CM_FEATURES, np.float32)
+    _check(lib().qsv_cm_block_features(n, kq, int(split), tile, _ints(qubits), _fptr(f)))
+    return f
+
+
+class CostModel:
+    """The fused-block cost model: an MLP whose forward, backward and Adam
+    steps run as HIP kernels on the device."""
+
+    def __init__(self, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().qsv_cm_create(seed, C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def nparams() -> int:
+        return lib().qsv_cm_nparams()
+
+    def close(self):
+        if self.h:
+            lib().qsv_cm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.nparams(), np.float32)
+        _check(lib().qsv_cm_get_params(self.h, _fptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams()
+        _check(lib().qsv_cm_set_params(self.h, _fptr(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, CM_FEATURES)
+        y = np.zeros(x.shape[0], np.float32)
+        _check(lib().qsv_cm_forward(self.h, _fptr(x), x.shape[0], _fptr(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, CM_FEATURES)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.nparams(), np.float32)
+        loss = C.c_double()
+        _check(lib().qsv_cm_gradients(self.h, _fptr(x), _fptr(t), x.shape[0], _fptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs: int, lr: float = 3e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, CM_FEATURES)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        _check(lib().qsv_cm_train(self.h, _fptr(x), _fptr(t), x.shape[0], epochs, lr, C.byref(loss)))
+        return loss.value
+
+    def save(self, path: str = DEFAULT_COSTMODEL):
+        self.get_params().tofile(path)
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_COSTMODEL) -> "CostModel":
+        m = cls(1)
+        m.set_params(np.fromfile(path, np.float32))
+        return m
 
 
 def fuse_plan(circ: "_circ.Circuit", max_k: int, exact: bool = False):

@@ -1,0 +1,61 @@
+"""The cost-model oracle (oracle/mlp_oracle.c) pinned before it checks the HIP
+kernels: its chain-rule gradient must equal central finite differences of its
+own loss, and its forward pass must equal a numpy restatement of the layer
+equations.  Also the block-feature function (host code in libqsv.so)."""
+import numpy as np
+
+import oracle as O
+from qpanda_amd import engine as E
+
+NP = 1505
+
+
+def params(seed):
+    g = np.random.default_rng(seed)
+    p = g.normal(0, 0.4, NP)
+    return p
+
+
+def data(seed, batch=37):
+    g = np.random.default_rng(seed)
+    return g.uniform(0, 1, (batch, 12)), g.normal(0, 1, batch)
+
+
+def test_forward_matches_numpy_layers():
+    p = params(1)
+    x, _ = data(2)
+    W1 = p[:384].reshape(32, 12); b1 = p[384:416]
+    W2 = p[416:1440].reshape(32, 32); b2 = p[1440:1472]
+    W3 = p[1472:1504]; b3 = p[1504]
+    h1 = np.maximum(x @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    y = h2 @ W3 + b3
+    assert np.abs(O.mlp_forward(p, x) - y).max() < 1e-12
+
+
+def test_gradient_matches_finite_differences():
+    p = params(3)
+    x, t = data(4)
+    g, loss = O.mlp_grad(p, x, t)
+    assert abs(loss - O.mlp_loss(p, x, t)) < 1e-14
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([[0, 383, 384, 415, 416, 1439, 1440, 1471, 1472, 1503, 1504],
+                          rng.integers(0, NP, 60)])
+    h = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (O.mlp_loss(pp, x, t) - O.mlp_loss(pm, x, t)) / (2 * h)
+        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_block_features():
+    f = E.block_features(30, 4, 1, 7, [3, 10, 20, 29])
+    assert f.shape == (12,)
+    assert f[1] == 1 and f[2] == 0 and f[3] == 1 and f[5] == 1
+    assert abs(f[7] - 0.25) < 1e-7            # one qubit inside the 7-bit run
+    assert abs(f[8] - 3 / 30) < 1e-7 and abs(f[9] - 29 / 30) < 1e-7
+    assert abs(f[10] - 0.25) < 1e-7           # one of the top three qubits
+    f5 = E.block_features(28, 5, 0, 8, [0, 1, 2, 3, 4])
+    assert f5[2] == 1 and f5[3] == 0 and f5[6] == 1 and f5[7] == 1.0

@@ -1,0 +1,106 @@
+"""The cost-model kernels (forward, backward, Adam on the device) against the
+FP64 oracle (oracle/mlp_oracle.c, itself pinned by finite differences), the
+engine's timing log, the learned scheduler's independence of results, and the
+trained network's measured gain over the greedy schedule it replaces."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from qpanda_amd import circuits as C
+from qpanda_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def batch(seed, n=300):
+    g = np.random.default_rng(seed)
+    x = g.uniform(0, 1, (n, 12)).astype(np.float32)
+    t = (np.sin(3 * x[:, 0]) + x[:, 1] * x[:, 2] - 0.5 * x[:, 7]).astype(np.float32)
+    return x, t
+
+
+def test_forward_matches_oracle():
+    m = E.CostModel(11)
+    x, _ = batch(1)
+    y = m.forward(x)
+    ref = O.mlp_forward(m.get_params().astype(np.float64), x.astype(np.float64))
+    assert np.abs(y - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_gradients_match_oracle():
+    m = E.CostModel(12)
+    x, t = batch(2)
+    g, loss = m.gradients(x, t)
+    gref, lref = O.mlp_grad(m.get_params().astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    scale = np.abs(gref).max()
+    err = np.abs(g - gref).max()
+    say("max |grad - oracle|", err, "scale", scale)
+    assert err < 1e-5 * scale + 1e-7
+    assert abs(loss - lref) < 1e-5 * max(1.0, lref)
+
+
+def test_training_on_device_reduces_loss():
+    m = E.CostModel(13)
+    x, t = batch(3, 600)
+    l0 = m.train(x, t, 0)
+    l1 = m.train(x, t, 800, 3e-3)
+    say("loss", l0, "->", l1)
+    assert l1 < 0.1 * l0
+    # the loss the device reports is the oracle's loss at the device's parameters
+    lref = O.mlp_loss(m.get_params().astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    assert abs(l1 - lref) < 1e-4 * max(1.0, lref)
+
+
+def test_timing_log_has_one_record_per_block():
+    n = 20
+    circ = C.random_layers(n, 8, 3, haar_frac=0.1)
+    with E.StateVector(n, E.C64) as sv:
+        st = sv.run(circ, fuse_k=4, profile=True)
+        f, ms = sv.timing_log()
+    assert f.shape == (st.blocks, 12) and ms.shape == (st.blocks,)
+    assert np.all(ms > 0)
+
+
+def test_learned_schedule_does_not_change_results():
+    n = 18
+    circ = C.random_layers(n, 12, 21, haar_frac=0.1)
+    m = E.CostModel(99)                      # untrained: arbitrary choices
+    with E.StateVector(n, E.C128) as ref, E.StateVector(n, E.C64) as a, E.StateVector(n, E.C64) as b:
+        ref.run(circ)
+        a.attach_costmodel(m)
+        st = a.run(circ, fuse_k=4, scheduler=1)
+        b.run(circ, fuse_k=4, scheduler=0)
+        fa, fb = ref.fidelity(a), ref.fidelity(b)
+    say("learned blocks", st.blocks, "fidelity", fa, "greedy", fb)
+    assert fa >= 1 - 1e-6 and fb >= 1 - 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
+def test_trained_model_gain_over_greedy():
+    m = E.CostModel.load()
+    rows = []
+    for circ in (C.random_layers(28, 24, 4242, haar_frac=0.1), C.hea(28, 6, 77), C.qft(26)):
+        n = circ.n
+        with E.StateVector(n, E.C64) as a, E.StateVector(n, E.C64) as b:
+            a.attach_costmodel(m)
+            tl, tg = [], []
+            for _ in range(3):
+                a.reset()
+                tl.append(a.run(circ, fuse_k=4, scheduler=1).device_ms)
+                b.reset()
+                tg.append(b.run(circ, fuse_k=4, scheduler=0).device_ms)
+            fid = a.fidelity(b)
+        rows.append((circ.name, np.median(tg), np.median(tl), fid))
+        say(f"{circ.name}: greedy k=4 {np.median(tg):.1f} ms, learned {np.median(tl):.1f} ms, "
+            f"gain {np.median(tg) / np.median(tl):.3f}x, fidelity {fid:.12f}")
+    for name, tg, tl, fid in rows:
+        assert tl <= 1.05 * tg, name
+        assert fid >= 1 - 1e-6
