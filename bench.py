@@ -47,7 +47,11 @@ def parse():
     p.add_argument("--haar", type=float, default=0.1)
     p.add_argument("--seed", type=int, default=2025)
     p.add_argument("--fuse-k", type=int, default=4)
-    p.add_argument("--scheduler", choices=["greedy", "learned"], default="greedy")
+    p.add_argument("--scheduler", choices=["greedy", "passes", "learned"], default="greedy",
+                   help="greedy: one block per sweep; passes: greedy multi-block sweeps; "
+                        "learned: cost-model schedule (needs a trained model)")
+    p.add_argument("--group", type=int, default=8, help="blocks per sweep for --scheduler passes")
+    p.add_argument("--costmodel", default=None, help="trained cost-model parameters (float32)")
     p.add_argument("--path", choices=["auto", "plain", "fused"], default="auto")
     p.add_argument("--no-plain", action="store_true", help="skip the FP64 comparison run")
     p.add_argument("--no-cpu", action="store_true")
@@ -161,7 +165,13 @@ def main():
     headline = "fused" if fused_ok else "plain"
     steps = a.steps if a.steps is not None else (3 if headline == "fused" else 1)
     warmup = a.warmup if a.warmup is not None else (1 if headline == "fused" else 0)
-    sched = 1 if a.scheduler == "learned" else 0
+    sched = {"greedy": 0, "learned": 1, "passes": 2}[a.scheduler]
+    cm = None
+    if sched == 1:
+        path = a.costmodel or E.DEFAULT_COSTMODEL
+        if not os.path.exists(path):
+            raise SystemExit(f"--scheduler learned needs a trained cost model ({path})")
+        cm = E.CostModel.load(path)
 
     plain = None
     result = {}
@@ -190,22 +200,26 @@ def main():
             plain = run_plain(True)
             sv_plain.sync()
         sv = E.StateVector(a.n, E.C64)
+        if cm is not None:
+            sv.attach_costmodel(cm)
         k = a.fuse_k
         for _ in range(warmup):
             sv.reset()
-            sv.run(circ, fuse_k=k, split=True, scheduler=sched)
+            sv.run(circ, fuse_k=k, split=True, scheduler=sched, group=a.group)
         sv.sync()
         t0 = time.perf_counter()
         stats = []
         for _ in range(steps):
             sv.reset()
-            stats.append(sv.run(circ, fuse_k=k, split=True, scheduler=sched, profile=True))
+            stats.append(sv.run(circ, fuse_k=k, split=True, scheduler=sched, group=a.group, profile=True))
         sv.sync()
         wall = time.perf_counter() - t0
         head_stats = stats
         if sv_plain is not None:
             result["fidelity_vs_fp64"] = sv_plain.fidelity(sv)
         result["fused"] = {"k": k, "split": True, "scheduler": a.scheduler,
+                           "launches": int(stats[-1].classes().get("fused", {}).get("launches", 0)),
+                           "plan_ms": stats[-1].plan_ms,
                            "blocks": int(stats[-1].blocks),
                            "device_ms": stats[-1].device_ms,
                            "predicted_ms": stats[-1].model_ms_pred}

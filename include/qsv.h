@@ -248,8 +248,13 @@ typedef struct qsv_run_opts {
     int profile;        /* 1: time every launch with events (per-class stats)   */
     int tile;           /* fused blocks: tile column bits 6, 7 or 8 (0 = 7);   */
                         /* the learned scheduler picks per block instead        */
-    int reserved[2];
+    int group;          /* scheduler 2: at most this many consecutive blocks    */
+                        /* share one HBM sweep (0 or 1 = one block per sweep)   */
+    int pass_tb;        /* scheduler 2: tile bits of such a pass (0 = minimal)  */
 } qsv_run_opts;
+/* opts.scheduler: 0 = greedy blocks at fuse_k, one sweep each;
+ *                 1 = learned (block size, grouping into sweeps, tile shapes);
+ *                 2 = greedy blocks grouped greedily into multi-block sweeps. */
 
 /* Kernel classes for per-launch profiling. */
 enum {
@@ -278,6 +283,8 @@ typedef struct qsv_run_stats {
     double  class_ms[QSV_K_CLASSES];
     double  class_bytes[QSV_K_CLASSES];
     double  class_flops[QSV_K_CLASSES];
+    double  plan_ms;        /* host time spent planning (fused path); the   */
+                            /* device time above starts after planning      */
 } qsv_run_stats;
 
 /* Greedy fusion plan of a gate list (host only, no device needed): blocks of at

@@ -249,9 +249,9 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
     Profiler prof(s, o.profile != 0);
     QSV_HIP(hipEventRecord(e0, s->stream));
     int64_t blocks = 0;
-    double pred_ms = 0;
+    double pred_ms = 0, plan_ms = 0;
     if (o.fuse_k >= 2) {
-        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof);
+        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof, e0, &plan_ms);
     } else {
         for (int64_t i = 0; i < ngates && !rc; i++) {
             rc = prof.begin();
@@ -281,6 +281,7 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
         stats->tie_resolutions = s->tie_resolutions - ties0;
         stats->device_ms = ms;
         stats->model_ms_pred = pred_ms;
+        stats->plan_ms = plan_ms;
         if (!rc) rc = prof.collect(stats);
     }
     if (!rc && prof.on() && o.fuse_k >= 2) {

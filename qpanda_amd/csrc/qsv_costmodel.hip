@@ -188,6 +188,173 @@ static std::vector<int> widened_qubits(const std::vector<int> &qs, int kq)
     return out;
 }
 
+void launch_features(int n, int split, const std::vector<const std::vector<int> *> &qsets, bool single, int tb,
+                     float *f)
+{
+    uint64_t u = 0;
+    int work = 0, five = 0;
+    for (auto *q : qsets) {
+        for (int x : *q) u |= uint64_t(1) << x;
+        work += 1 << q->size();
+        five += q->size() == 5;
+    }
+    uint64_t lmask = u;
+    for (int b = 0; __builtin_popcountll(lmask) < tb && b < 64; b++) lmask |= uint64_t(1) << b;
+    int run = 0;
+    while (run < 64 && (lmask >> run & 1)) run++;
+    const int nu = __builtin_popcountll(u);
+    int mn = 64, mx = -1, top = 0;
+    for (int b = 0; b < 64; b++)
+        if (u >> b & 1) {
+            mn = std::min(mn, b);
+            mx = std::max(mx, b);
+            top += b >= n - 3;
+        }
+    const int G = (int)qsets.size();
+    f[0] = n / 34.0f;
+    f[1] = single ? 1.0f : 0.0f;
+    f[2] = G / 8.0f;
+    f[3] = work / 128.0f;
+    f[4] = split ? 1.0f : 0.0f;
+    f[5] = (tb - 10) / 5.0f;
+    f[6] = (float)std::min(run, tb) / tb;
+    f[7] = nu / 14.0f;
+    f[8] = (float)five / G;
+    f[9] = (float)mn / n;
+    f[10] = (float)mx / n;
+    f[11] = nu ? (float)top / nu : 0.0f;
+}
+
+// Single-block tile column bits that fit this state (6..8), default 7.
+static int default_tc(int n, int kq)
+{
+    return std::max(6, std::min(kDefaultTileCols, n - kq));
+}
+
+int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_run_opts &o,
+                   std::vector<Launch> &launches, double *pred_total)
+{
+    launches.clear();
+    *pred_total = 0;
+    const int n = s->n;
+    const bool learned = o.scheduler == 1 && s->cm;
+    const int gmax = learned ? kMaxPassBlocks : std::max(1, std::min(o.group, kMaxPassBlocks));
+    size_t i = 0;
+    while (i < plan.size()) {
+        if (plan[i].barrier >= 0) {
+            Launch L;
+            L.first = i;
+            launches.push_back(L);
+            i++;
+            continue;
+        }
+        // segment of consecutive blocks up to the next barrier
+        size_t j = i;
+        while (j < plan.size() && plan[j].barrier < 0) j++;
+        const size_t m = j - i;
+        auto mfma_ok = [&](size_t b) { return plan[b].qubits.size() >= 4 && (int)plan[b].qubits.size() + 6 <= n; };
+        if (!learned) {
+            size_t b = i;
+            while (b < j) {
+                Launch L;
+                L.first = b;
+                L.count = 1;
+                L.tc = mfma_ok(b) ? (o.tile ? o.tile : default_tc(n, (int)plan[b].qubits.size())) : 0;
+                if (mfma_ok(b) && (int)plan[b].qubits.size() + L.tc > n) L.tc = default_tc(n, (int)plan[b].qubits.size());
+                std::vector<const std::vector<int> *> q = {&plan[b].qubits};
+                while (mfma_ok(b) && gmax > 1 && b + L.count < j && L.count < gmax && mfma_ok(b + L.count)) {
+                    q.push_back(&plan[b + L.count].qubits);
+                    int tb = pass_tile_bits(q, n);
+                    if (tb < 0) {
+                        q.pop_back();
+                        break;
+                    }
+                    L.count++;
+                }
+                if (L.count > 1) {
+                    L.tb = pass_tile_bits(q, n);
+                    if (o.pass_tb > L.tb && o.pass_tb <= std::min(14, n)) L.tb = o.pass_tb;
+                }
+                launches.push_back(L);
+                b += L.count;
+            }
+            i = j;
+            continue;
+        }
+        // learned: dynamic programming over the segment with predicted times
+        struct Cand { size_t start; int count, tc, tb; };
+        std::vector<Cand> cands;
+        std::vector<float> feats;
+        for (size_t e = 0; e < m; e++) {            // candidate launches ending at block i+e
+            const size_t last = i + e;
+            if (!mfma_ok(last)) {
+                cands.push_back({last, 1, 0, 0});
+                float f[F] = {0};
+                feats.insert(feats.end(), f, f + F);
+                continue;
+            }
+            const int kq = (int)plan[last].qubits.size();
+            std::vector<const std::vector<int> *> q1 = {&plan[last].qubits};
+            for (int tc = 6; tc <= 8; tc++) {
+                if (kq + tc > n) continue;
+                float f[F];
+                launch_features(n, o.split, q1, true, kq + tc, f);
+                cands.push_back({last, 1, tc, 0});
+                feats.insert(feats.end(), f, f + F);
+            }
+            std::vector<const std::vector<int> *> q = q1;
+            for (int g = 2; g <= kMaxPassBlocks && (size_t)g <= e + 1; g++) {
+                const size_t st = last + 1 - g;
+                if (!mfma_ok(st)) break;
+                q.insert(q.begin(), &plan[st].qubits);
+                const int tbmin = pass_tile_bits(q, n);
+                if (tbmin < 0) break;
+                for (int tb = tbmin; tb <= std::min(14, n); tb++) {
+                    float f[F];
+                    launch_features(n, o.split, q, false, tb, f);
+                    cands.push_back({st, g, 0, tb});
+                    feats.insert(feats.end(), f, f + F);
+                }
+            }
+        }
+        std::vector<float> y(cands.size());
+        int rc = qsv_cm_forward(s->cm, feats.data(), (int64_t)cands.size(), y.data());
+        if (rc) return rc;
+        const double scale = std::ldexp(1.0, n - 30);
+        std::vector<double> dp(m + 1, std::numeric_limits<double>::infinity());
+        std::vector<int> from(m + 1, -1);
+        dp[0] = 0;
+        // candidates are grouped by their end block, in increasing order
+        for (size_t c = 0; c < cands.size(); c++) {
+            const Cand &cd = cands[c];
+            const size_t end = cd.start + cd.count - i;      // dp index after this launch
+            const size_t beg = cd.start - i;
+            const double ms = (cd.tc == 0 && cd.tb == 0) ? 0.0 : std::exp((double)y[c]) * scale;
+            if (dp[beg] + ms < dp[end]) {
+                dp[end] = dp[beg] + ms;
+                from[end] = (int)c;
+            }
+        }
+        std::vector<Launch> seg;
+        for (size_t e = m; e > 0;) {
+            const Cand &cd = cands[from[e]];
+            Launch L;
+            L.first = cd.start;
+            L.count = cd.count;
+            L.tc = cd.tc;
+            L.tb = cd.tb;
+            L.pred_ms = dp[e] - dp[cd.start - i];
+            seg.push_back(L);
+            e = cd.start - i;
+        }
+        std::reverse(seg.begin(), seg.end());
+        launches.insert(launches.end(), seg.begin(), seg.end());
+        *pred_total += dp[m];
+        i = j;
+    }
+    return QSV_OK;
+}
+
 int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *mats, const qsv_run_opts &o,
                  std::vector<PlanItem> &plan, SchedChoice *choice)
 {
@@ -195,7 +362,6 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
         choice->fuse_k = o.fuse_k;
         return plan_greedy(gates, ng, mats, o.fuse_k, o.exact, plan);
     }
-    const int tcs[3] = {6, 7, 8};
     double best_total = std::numeric_limits<double>::infinity();
     int best_k = o.fuse_k;
     std::vector<PlanItem> best;
@@ -203,44 +369,22 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
         std::vector<PlanItem> cand;
         int rc = plan_greedy(gates, ng, mats, k, o.exact, cand);
         if (rc) return rc;
-        // features of every block at every tile shape that fits
-        std::vector<float> feats;
-        std::vector<std::pair<size_t, int>> who;      // (block, tc)
-        for (size_t i = 0; i < cand.size(); i++) {
-            if (cand[i].barrier >= 0) continue;
-            const int kq = std::max(4, (int)cand[i].qubits.size());
-            const std::vector<int> wq = widened_qubits(cand[i].qubits, kq);
-            for (int tc : tcs) {
-                if (s->n < kq + tc) continue;
-                float f[F];
-                qsv_cm_block_features(s->n, kq, o.split, tc, wq.data(), f);
-                feats.insert(feats.end(), f, f + F);
-                who.emplace_back(i, tc);
+        for (PlanItem &it : cand)
+            if (it.barrier < 0 && s->n >= std::max(4, (int)it.qubits.size()) + 6) {
+                // qubit list as the run will widen it (the matrix is widened later)
+                it.qubits = widened_qubits(it.qubits, std::max(4, (int)it.qubits.size()));
             }
-        }
-        std::vector<float> y(who.size());
-        if (!who.empty()) {
-            rc = qsv_cm_forward(s->cm, feats.data(), (int64_t)who.size(), y.data());
-            if (rc) return rc;
-        }
-        const double scale = std::ldexp(1.0, s->n - 30);
-        std::vector<double> best_ms(cand.size(), std::numeric_limits<double>::infinity());
-        for (size_t j = 0; j < who.size(); j++) {
-            const double ms = std::exp((double)y[j]) * scale;
-            PlanItem &it = cand[who[j].first];
-            if (ms < best_ms[who[j].first]) {
-                best_ms[who[j].first] = ms;
-                it.tile = who[j].second;
-                it.pred_ms = ms;
-            }
-        }
+        std::vector<Launch> launches;
         double total = 0;
-        for (size_t i = 0; i < cand.size(); i++)
-            if (cand[i].barrier < 0) total += std::isfinite(best_ms[i]) ? best_ms[i] : 0;
+        rc = group_launches(s, cand, o, launches, &total);
+        if (rc) return rc;
         if (total < best_total) {
             best_total = total;
             best_k = k;
-            best.swap(cand);
+            // re-plan to keep the matrices consistent with their qubit lists
+            best.clear();
+            rc = plan_greedy(gates, ng, mats, k, o.exact, best);
+            if (rc) return rc;
         }
     }
     plan.swap(best);
@@ -259,28 +403,10 @@ int qsv_cm_nparams(void) { return kParams; }
 
 int qsv_cm_block_features(int n, int kq, int split, int tile, const int *qubits, float *f)
 {
-    if (!qubits || !f || n < 1 || kq < 1 || kq > 5) return fail(QSV_ERR_ARG, "bad feature arguments");
-    int lo = 0, top = 0, above = 0, mn = 64, mx = -1;
-    for (int j = 0; j < kq; j++) {
-        const int q = qubits[j];
-        lo += q < tile;
-        top += q >= n - 3;
-        above += q >= tile && q < tile + 6;
-        mn = std::min(mn, q);
-        mx = std::max(mx, q);
-    }
-    f[0] = n / 34.0f;
-    f[1] = kq == 4;
-    f[2] = kq == 5;
-    f[3] = split ? 1.0f : 0.0f;
-    f[4] = tile == 6;
-    f[5] = tile == 7;
-    f[6] = tile == 8;
-    f[7] = (float)lo / kq;
-    f[8] = (float)mn / n;
-    f[9] = (float)mx / n;
-    f[10] = (float)top / kq;
-    f[11] = (float)above / kq;
+    if (!qubits || !f || n < 1 || kq < 1 || kq > 5 || tile < 6 || tile > 8) return fail(QSV_ERR_ARG, "bad feature arguments");
+    std::vector<int> q(qubits, qubits + kq);
+    std::vector<const std::vector<int> *> qs = {&q};
+    launch_features(n, split, qs, true, kq + tile, f);
     return QSV_OK;
 }
 

@@ -17,8 +17,11 @@
 //          K-step s it feeds amplitude s of its 32 columns (re in lanes 0-31,
 //          im in lanes 32-63) as the B operand of v_mfma_f32_32x32x2f32, with M's
 //          columns 2s, 2s+1 as A (held in registers for the whole tile);
-//          split = 1 adds a second product with M_lo = fp32(M - fp32(M)), both
-//          accumulated in FP32;
+//          split = 1 adds the correction M_lo X with M_lo = M - fp32(M): its
+//          terms are ~2^-24 of the result, so it runs on the BF16 MFMA
+//          (v_mfma_f32_32x32x16_bf16, 16x the FP32 rate) with both operands
+//          rounded to bf16 (relative error of the correction <= 2^-8, i.e.
+//          ~2^-32 of the result), accumulated into the same FP32 registers;
 //   store: results go back to the planes, then 16-byte coalesced stores.
 // Per tile the kernel moves 2 * 8 * 2^(k+7) bytes and performs
 // 2 * (2d)^2 * 128 flops per product, so k = 4 sits near the FP32 MFMA ridge and
@@ -26,13 +29,17 @@
 //
 // MFMA 32x32x2 f32 operand layout (one block): A[i][k] in lane i + 32k,
 // B[k][j] in lane j + 32k, D[i][j] in lane j + 32*((i/4)%2), register
-// (i/8)*4 + i%4.
+// (i/8)*4 + i%4.  32x32x16 bf16: A[i][k] in lane i + 32*(k/8), element k%8;
+// B[k][j] in lane j + 32*(k/8), element k%8; same D layout.
 #include "qsv_internal.h"
 #include "qsv_sched.h"
+
+#include <cstring>
 
 namespace qsv {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 struct FusedArgs {
     uint64_t off_high[32];   // address offset of the tile bits above the low run
@@ -62,15 +69,20 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     const uint64_t base = expand_index(tile, fa.tile_ins);
 
     // A operands (the matrix), operand-ready layout: mops[(h*KSTEPS + s)*64 + lane]
+    constexpr int LOBLK = D / 8;             // bf16 K-blocks (16 real rows = 8 amplitudes)
     float ahi[HALVES][KSTEPS];
-    float alo[SPLIT ? HALVES : 1][SPLIT ? KSTEPS : 1];
+    bf16x8 alo[SPLIT ? HALVES : 1][SPLIT ? LOBLK : 1];
 #pragma unroll
     for (int h = 0; h < HALVES; h++)
 #pragma unroll
-        for (int s = 0; s < KSTEPS; s++) {
-            ahi[h][s] = mops[(h * KSTEPS + s) * 64 + lane];
-            if (SPLIT) alo[SPLIT ? h : 0][SPLIT ? s : 0] = mops[(HALVES * KSTEPS + h * KSTEPS + s) * 64 + lane];
-        }
+        for (int s = 0; s < KSTEPS; s++) ahi[h][s] = mops[(h * KSTEPS + s) * 64 + lane];
+    if (SPLIT) {
+        const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(mops + HALVES * KSTEPS * 64);
+#pragma unroll
+        for (int h = 0; h < HALVES; h++)
+#pragma unroll
+            for (int sb = 0; sb < LOBLK; sb++) alo[SPLIT ? h : 0][SPLIT ? sb : 0] = lo[(h * LOBLK + sb) * 64 + lane];
+    }
 
     // load the tile: element pairs (e, e+1) are adjacent in memory (bit 0 is in L)
 #pragma unroll
@@ -102,9 +114,21 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
         const int e = fa.erow[s] | ecol;
         const float b = bplane[e];
 #pragma unroll
-        for (int h = 0; h < HALVES; h++) {
-            acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], b, acc[h], 0, 0, 0);
-            if (SPLIT) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(alo[SPLIT ? h : 0][SPLIT ? s : 0], b, acc[h], 0, 0, 0);
+        for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], b, acc[h], 0, 0, 0);
+    }
+    if (SPLIT) {
+#pragma unroll
+        for (int sb = 0; sb < LOBLK; sb++) {
+            // rows 16sb + 8(lane/32) + e: amplitude 8sb + 4(lane/32) + e/2, part e%2
+            bf16x8 bl;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int idx = fa.erow[8 * sb + 4 * (lane >> 5) + (e >> 1)] | ecol;
+                bl[e] = (__bf16)((e & 1) ? pim[idx] : pre[idx]);
+            }
+#pragma unroll
+            for (int h = 0; h < HALVES; h++)
+                acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[SPLIT ? h : 0][SPLIT ? sb : 0], bl, acc[h], 0, 0, 0);
         }
     }
     // all B reads of this wave precede its writes (data dependence); other
@@ -131,7 +155,19 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
 }
 
 // Host-side: operand-ready matrices for a block.  u: d x d complex doubles
-// (row bit j = bit(qubits[j])).  out: (split ? 2 : 1) * HALVES * D * 64 floats.
+// (row bit j = bit(qubits[j])).  Layout: HALVES * D * 64 floats of M_hi
+// (value M[32h + l%32][2s + l/32] at (h*D + s)*64 + l), then, with split,
+// HALVES * (D/8) * 64 * 8 bf16 of M_lo (value M_lo[32h + l%32][16sb + 8(l/32) + e]
+// at ((h*(D/8) + sb)*64 + l)*8 + e).
+static uint16_t to_bf16(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t lsb = (x >> 16) & 1;          // round to nearest even
+    x += 0x7FFFu + lsb;
+    return (uint16_t)(x >> 16);
+}
+
 void fused_operands(int kq, const double *u, int split, float *out)
 {
     const int d = 1 << kq, R = 2 * d, halves = R / 32;
@@ -143,19 +179,25 @@ void fused_operands(int kq, const double *u, int split, float *out)
     };
     for (int h = 0; h < halves; h++)
         for (int s = 0; s < d; s++)
-            for (int l = 0; l < 64; l++) {
-                const int i = 32 * h + (l & 31), j = 2 * s + (l >> 5);
-                const double v = M(i, j);
-                const float hi = (float)v;
-                out[(h * d + s) * 64 + l] = hi;
-                if (split) out[(halves * d + h * d + s) * 64 + l] = (float)(v - (double)hi);
-            }
+            for (int l = 0; l < 64; l++)
+                out[(h * d + s) * 64 + l] = (float)M(32 * h + (l & 31), 2 * s + (l >> 5));
+    if (!split) return;
+    uint16_t *lo = reinterpret_cast<uint16_t *>(out + halves * d * 64);
+    const int lob = d / 8;
+    for (int h = 0; h < halves; h++)
+        for (int sb = 0; sb < lob; sb++)
+            for (int l = 0; l < 64; l++)
+                for (int e = 0; e < 8; e++) {
+                    const double v = M(32 * h + (l & 31), 16 * sb + 8 * (l >> 5) + e);
+                    lo[((h * lob + sb) * 64 + l) * 8 + e] = to_bf16((float)(v - (double)(float)v));
+                }
 }
 
 size_t fused_operand_floats(int kq, int split)
 {
-    const int d = 1 << kq;
-    return (size_t)(split ? 2 : 1) * ((2 * d) / 32) * d * 64;
+    const int d = 1 << kq, halves = (2 * d) / 32;
+    const size_t hi = (size_t)halves * d * 64;
+    return hi + (split ? hi / 2 : 0);            // bf16 lo part: half the bytes of the hi part
 }
 
 template <int KQ, int TC>

@@ -18,6 +18,7 @@
 #include "qsv_sched.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -324,9 +325,11 @@ void widen_to(PlanItem &it, int kq)
 
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &o, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms,
-              Profiler *prof)
+              Profiler *prof, hipEvent_t start, double *plan_ms)
 {
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "the fused path runs on c64 states");
+    const auto t_plan = std::chrono::steady_clock::now();
+    if (o.tile != 0 && (o.tile < 6 || o.tile > 8)) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
     std::vector<PlanItem> plan;
     int rc;
     SchedChoice choice;
@@ -337,25 +340,36 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
         choice.fuse_k = o.fuse_k;
     }
     if (rc) return rc;
-    if (o.tile != 0 && (o.tile < 6 || o.tile > 8)) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
     const int kq_min = 4;
-    // operands for every block in one upload; blocks whose tile does not fit
-    // the state run on the FP64 block kernel instead
-    std::vector<size_t> op_off(plan.size(), 0);
+    // widen to >= 4 qubits where an MFMA tile fits; other blocks use the FP64 kernel
     std::vector<char> mfma(plan.size(), 0);
-    std::vector<float> ops;
     for (size_t i = 0; i < plan.size(); i++) {
         PlanItem &it = plan[i];
         if (it.barrier >= 0) continue;
-        if (o.scheduler != 1) it.tile = o.tile;
-        const int tc = it.tile ? it.tile : kDefaultTileCols;
         const int kq = std::max(kq_min, (int)it.qubits.size());
-        if (s->n < kq + tc) continue;
+        if (s->n < kq + 6) continue;
         widen_to(it, kq_min);
         mfma[i] = 1;
+    }
+    std::vector<Launch> launches;
+    double pred = 0;
+    if (o.scheduler == 1 && s->cm) {
+        rc = group_launches(s, plan, o, launches, &pred);
+    } else {
+        qsv_run_opts g = o;
+        if (o.scheduler != 2) g.group = 1;
+        rc = group_launches(s, plan, g, launches, &pred);
+    }
+    if (rc) return rc;
+    // operands for every MFMA block in one upload
+    std::vector<size_t> op_off(plan.size(), 0);
+    std::vector<float> ops;
+    for (size_t i = 0; i < plan.size(); i++) {
+        if (!mfma[i]) continue;
+        const int kq = (int)plan[i].qubits.size();
         op_off[i] = ops.size();
         ops.resize(ops.size() + fused_operand_floats(kq, o.split));
-        fused_operands(kq, it.u.data(), o.split, ops.data() + op_off[i]);
+        fused_operands(kq, plan[i].u.data(), o.split, ops.data() + op_off[i]);
     }
     float *dops = nullptr;
     if (!ops.empty()) {
@@ -369,17 +383,19 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
         }
     }
     const double N = (double)s->dim;
-    double pred = 0;
     int64_t nblocks = 0;
+    *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
+    QSV_HIP(hipEventRecord(start, s->stream));       // device time starts after planning
     rc = QSV_OK;
     if (prof->on()) {
         s->tlog_feats.clear();
         s->tlog_ms.clear();
     }
-    for (size_t i = 0; i < plan.size() && !rc; i++) {
-        const PlanItem &it = plan[i];
-        if (it.barrier >= 0) {
-            const qsv_gate &r = gates[it.barrier];
+    for (const Launch &L : launches) {
+        if (rc) break;
+        const PlanItem &it0 = plan[L.first];
+        if (it0.barrier >= 0) {
+            const qsv_gate &r = gates[it0.barrier];
             // one draw for the measurement, one more for a Reset's NOT on outcome 1
             double draw = 1.0;
             if (rng) {
@@ -397,26 +413,41 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
             if (!rc) rc = prof->end(QSV_K_READOUT, 2 * 8 * N, 0);
             continue;
         }
-        if (rng) *rng = qsv_pm_jump(*rng, (uint64_t)it.ngates);
-        const int kq = (int)it.qubits.size();
-        if ((rc = prof->begin())) break;
-        if (mfma[i]) {
-            const int tc = it.tile ? it.tile : kDefaultTileCols;
-            rc = launch_fused_dev(s, it.qubits.data(), kq, dops + op_off[i], o.split, tc);
-            int64_t tag = -1;
-            if (!rc && prof->on()) {
-                float f[QSV_CM_FEATURES];
-                qsv_cm_block_features(s->n, kq, o.split, tc, it.qubits.data(), f);
-                tag = (int64_t)(s->tlog_feats.size() / QSV_CM_FEATURES);
-                s->tlog_feats.insert(s->tlog_feats.end(), f, f + QSV_CM_FEATURES);
-            }
-            if (!rc) rc = prof->end(QSV_K_FUSED, 2 * 8 * N, 8.0 * N * (1 << kq) * (o.split ? 2 : 1), tag);
-        } else {
-            rc = launch_block(s, it.qubits.data(), kq, it.u.data(), 0);
-            if (!rc) rc = prof->end(QSV_K_BLOCK64, 2 * 8 * N, 8.0 * N * (1 << kq));
+        double flops = 0;
+        int64_t gates_here = 0;
+        std::vector<const std::vector<int> *> qsets;
+        std::vector<const float *> mp;
+        for (int j = 0; j < L.count; j++) {
+            const PlanItem &it = plan[L.first + j];
+            gates_here += it.ngates;
+            qsets.push_back(&it.qubits);
+            mp.push_back(dops + op_off[L.first + j]);
+            flops += 8.0 * N * (1 << it.qubits.size());     // FP32 MFMA product (the BF16 correction is extra)
         }
-        pred += it.pred_ms;
-        nblocks++;
+        if (rng) *rng = qsv_pm_jump(*rng, (uint64_t)gates_here);
+        if ((rc = prof->begin())) break;
+        int cls = QSV_K_FUSED;
+        bool single = L.count == 1;
+        int tb = L.tb;
+        if (single && !mfma[L.first]) {
+            rc = launch_block(s, it0.qubits.data(), (int)it0.qubits.size(), it0.u.data(), 0);
+            cls = QSV_K_BLOCK64;
+        } else if (single) {
+            const int tc = L.tc ? L.tc : kDefaultTileCols;
+            tb = (int)it0.qubits.size() + tc;
+            rc = launch_fused_dev(s, it0.qubits.data(), (int)it0.qubits.size(), mp[0], o.split, tc);
+        } else {
+            rc = launch_pass_dev(s, qsets, mp, o.split, tb);
+        }
+        int64_t tag = -1;
+        if (!rc && prof->on() && cls == QSV_K_FUSED) {
+            float f[QSV_CM_FEATURES];
+            launch_features(s->n, o.split, qsets, single, tb, f);
+            tag = (int64_t)(s->tlog_feats.size() / QSV_CM_FEATURES);
+            s->tlog_feats.insert(s->tlog_feats.end(), f, f + QSV_CM_FEATURES);
+        }
+        if (!rc) rc = prof->end(cls, 2 * 8 * N, flops, tag);
+        nblocks += L.count;
     }
     if (dops) {
         (void)hipStreamSynchronize(s->stream);

@@ -35,6 +35,6 @@ private:
 // Fuse `gates` into blocks of at most opts.fuse_k qubits and apply them.
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &opts, int32_t *rng, int *outcomes, int64_t *blocks,
-              double *pred_ms, Profiler *prof);
+              double *pred_ms, Profiler *prof, hipEvent_t start, double *plan_ms);
 
 }  // namespace qsv

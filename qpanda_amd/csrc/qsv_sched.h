@@ -40,4 +40,30 @@ size_t fused_operand_floats(int kq, int split);
 constexpr int kDefaultTileCols = 7;
 int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split, int tc);
 
+// Multi-block passes (qsv_pass.hip).
+constexpr int kMaxPassBlocks = 8;
+int pass_tile_bits(const std::vector<const std::vector<int> *> &qsets, int n);
+int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
+                    const std::vector<const float *> &mops, int split, int tb);
+
+// One device launch of a fused run: plan items [first, first + count) (all
+// blocks) in one sweep.  count == 1 uses the single-block kernel with tile
+// column bits `tc`; count > 1 uses the pass kernel with `tb` tile bits.
+struct Launch {
+    size_t first = 0;
+    int count = 1;
+    int tc = 0;
+    int tb = 0;
+    double pred_ms = 0;
+};
+
+// Cost-model features of one launch (single block: tb = kq + tc).
+void launch_features(int n, int split, const std::vector<const std::vector<int> *> &qsets, bool single,
+                     int tb, float *f);
+
+// Group a block plan into launches.  scheduler 0: one block per launch;
+// 2: greedy maximal groups (opts.group, opts.pass_tb); 1: cost-model DP.
+int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_run_opts &o,
+                   std::vector<Launch> &launches, double *pred_total);
+
 }  // namespace qsv

@@ -42,7 +42,7 @@ class QsvError(RuntimeError):
 class RunOpts(C.Structure):
     _fields_ = [("exact", C.c_int), ("fuse_k", C.c_int), ("split", C.c_int),
                 ("scheduler", C.c_int), ("profile", C.c_int), ("tile", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("group", C.c_int), ("pass_tb", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -50,7 +50,7 @@ class RunStats(C.Structure):
                 ("tie_resolutions", C.c_int64), ("device_ms", C.c_double),
                 ("model_ms_pred", C.c_double), ("class_launches", C.c_int64 * 8),
                 ("class_ms", C.c_double * 8), ("class_bytes", C.c_double * 8),
-                ("class_flops", C.c_double * 8)]
+                ("class_flops", C.c_double * 8), ("plan_ms", C.c_double)]
 
     CLASS_NAMES = ["dense1", "diag", "perm", "phase", "block64", "fused", "readout", "other"]
 
@@ -309,10 +309,13 @@ class StateVector:
         return out, rs.value, ties.value
 
     def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=True, scheduler=0,
-            rng_state=None, outcomes=None, profile=False, tile=0) -> RunStats:
+            rng_state=None, outcomes=None, profile=False, tile=0, group=0, pass_tb=0) -> RunStats:
+        """scheduler: 0 greedy blocks (one sweep each), 1 learned (needs an attached
+        cost model), 2 greedy blocks grouped into multi-block sweeps of <= group."""
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
-        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile))
+        opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile),
+                       int(group), int(pass_tb))
         st = RunStats()
         rs = C.c_int32(rng_state if rng_state is not None else 1)
         self._last_rng = None

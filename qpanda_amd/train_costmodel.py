@@ -1,18 +1,21 @@
-"""Train the fused-block cost model from timings the engine records itself.
+"""Train the fused-path cost model from timings the engine records itself.
 
     python -m qpanda_amd.train_costmodel [--out qpanda_amd/costmodel_params.f32]
                                          [--report report.json] [--quick]
 
 1. Collect: seeded circuits (random layers, QFT, hardware-efficient ansatz) at
-   several qubit counts run on the fused path at every block size (k = 2..5),
-   both split settings and every tile shape, with per-launch event timing; the
-   engine's timing log gives one (features, ms) record per block launch.
+   several qubit counts run on the fused path under every decision the
+   scheduler can make -- block size k = 2..5, split on / off, single-block
+   launches at every tile shape (6, 7, 8 column bits), and multi-block sweeps of
+   up to 2, 3, 4 or 8 blocks at minimal / 13 / 14 tile bits -- with per-launch
+   event timing; the engine's timing log gives one (features, ms) record per
+   launch.
 2. Train: full-batch Adam on the device (the engine's own forward / backward
    kernels), target log(ms * 2^(30 - n)).
-3. Evaluate on held-out circuits: per-block prediction error, and the measured
-   time of the learned schedule against the greedy k = 4 schedule (the method
-   without the model) on the same circuits; the circuit results of the two
-   schedules agree to FP32 rounding.
+3. Evaluate on held-out circuits: per-launch prediction error, and the measured
+   device time of the learned schedule against the greedy k = 4 schedule (one
+   block per sweep; the method without the model) and against greedy maximal
+   grouping; the states of all schedules agree to FP32 rounding.
 """
 from __future__ import annotations
 
@@ -27,6 +30,11 @@ import numpy as np
 from . import circuits as C
 from . import engine as E
 
+SINGLE = [dict(fuse_k=k, split=s, tile=t, scheduler=0) for k in (2, 3, 4, 5) for s in (False, True)
+          for t in (6, 7, 8)]
+PASSES = [dict(fuse_k=k, split=s, scheduler=2, group=g, pass_tb=tb) for k in (3, 4, 5) for s in (False, True)
+          for g in (2, 3, 4, 8) for tb in (0, 13, 14)]
+
 
 def workloads(ns, seed0, depth):
     out = []
@@ -39,7 +47,7 @@ def workloads(ns, seed0, depth):
 
 
 def collect(ns, seed0=7, depth=6, log=print):
-    """Run every workload at every (k, split, tile); return features, log-targets."""
+    """Run every workload under every configuration; return features, log-targets."""
     feats, targ = [], []
     by_n = {}
     for circ in workloads(ns, seed0, depth):
@@ -47,22 +55,20 @@ def collect(ns, seed0=7, depth=6, log=print):
     for n, circs in sorted(by_n.items()):
         with E.StateVector(n, E.C64) as sv:
             for circ in circs:
-                for k in (2, 3, 4, 5):
-                    for split in (0, 1):
-                        for tile in (6, 7, 8):
-                            sv.reset()
-                            sv.run(circ, fuse_k=k, split=bool(split), tile=tile)        # warm
-                            sv.reset()
-                            sv.run(circ, fuse_k=k, split=bool(split), tile=tile, profile=True)
-                            f, ms = sv.timing_log()
-                            keep = ms > 0
-                            feats.append(f[keep])
-                            targ.append(np.log(ms[keep].astype(np.float64) * 2.0 ** (30 - n)))
-            log(f"collected n={n}: {sum(x.shape[0] for x in feats)} records so far")
+                for cfg in SINGLE + PASSES:
+                    sv.reset()
+                    sv.run(circ, **cfg)                           # warm
+                    sv.reset()
+                    sv.run(circ, profile=True, **cfg)
+                    f, ms = sv.timing_log()
+                    keep = ms > 0
+                    feats.append(f[keep])
+                    targ.append(np.log(ms[keep].astype(np.float64) * 2.0 ** (30 - n)))
+        log(f"collected n={n}: {sum(x.shape[0] for x in feats)} records so far")
     return np.concatenate(feats).astype(np.float32), np.concatenate(targ).astype(np.float32)
 
 
-def train(x, t, epochs=3000, lr=3e-3, seed=1, log=print):
+def train(x, t, epochs=4000, lr=3e-3, seed=1, log=print):
     m = E.CostModel(seed)
     p = m.get_params()
     p[-1] = float(np.mean(t))                    # output bias at the mean target
@@ -73,41 +79,54 @@ def train(x, t, epochs=3000, lr=3e-3, seed=1, log=print):
     return m, loss0, loss
 
 
+METHODS = (("greedy_k4", dict(fuse_k=4, scheduler=0)),
+           ("passes_k4", dict(fuse_k=4, scheduler=2, group=8)),
+           ("learned", dict(fuse_k=4, scheduler=1)))
+
+
+def compare(m, circ, reps=3, split=True):
+    """Median device ms of each method on one circuit, plus fidelities to greedy."""
+    n = circ.n
+    res = {}
+    with E.StateVector(n, E.C64) as ref, E.StateVector(n, E.C64) as sv:
+        ref.run(circ, fuse_k=4, split=split)
+        sv.attach_costmodel(m)
+        for name, kw in METHODS:
+            times, plan = [], []
+            for rep in range(reps):
+                sv.reset()
+                st = sv.run(circ, split=split, profile=(rep == reps - 1), **kw)
+                times.append(st.device_ms)
+                plan.append(st.plan_ms)
+            r = {"ms": float(np.median(times)), "launches": int(st.classes().get("fused", {}).get("launches", 0)),
+                 "blocks": int(st.blocks), "plan_ms": float(np.median(plan)), "fidelity": sv.fidelity(ref)}
+            if name == "learned":
+                f, ms = sv.timing_log()
+                pred = np.exp(m.forward(f).astype(np.float64)) * 2.0 ** (n - 30)
+                r["launch_mape"] = float(np.mean(np.abs(pred - ms) / ms))
+                r["predicted_ms"] = float(st.model_ms_pred)
+            res[name] = r
+    return res
+
+
 def evaluate(m, ns=(26, 28, 30), seed0=1000, depth=24, log=print):
-    """Held-out circuits: per-block error and learned vs greedy measured time."""
     rows = []
     for n in ns:
         circs = [C.random_layers(n, depth, seed0 + n, haar_frac=0.1), C.hea(n, 6, seed0 + 50 + n)]
         if n <= 28:
             circs.append(C.qft(n))
-        with E.StateVector(n, E.C64) as sv, E.StateVector(n, E.C64) as sv2:
-            sv.attach_costmodel(m)
-            for circ in circs:
-                res = {}
-                for name, kw in (("greedy_k4", dict(fuse_k=4, scheduler=0)),
-                                 ("learned", dict(fuse_k=4, scheduler=1))):
-                    target = sv if name == "learned" else sv2
-                    times = []
-                    for rep in range(3):
-                        target.reset()
-                        st = target.run(circ, split=True, profile=(rep == 2), **kw)
-                        times.append(st.device_ms)
-                    res[name] = {"ms": float(np.median(times)), "blocks": int(st.blocks),
-                                 "predicted_ms": float(st.model_ms_pred)}
-                    if name == "learned":
-                        f, ms = target.timing_log()
-                        pred = np.exp(m.forward(f).astype(np.float64)) * 2.0 ** (n - 30)
-                        res[name]["block_mape"] = float(np.mean(np.abs(pred - ms) / ms))
-                fid = sv.fidelity(sv2)
-                row = {"circuit": circ.name or f"n{n}", "n": n, "gates": circ.ngates,
-                       "greedy_ms": res["greedy_k4"]["ms"], "learned_ms": res["learned"]["ms"],
-                       "learned_blocks": res["learned"]["blocks"], "greedy_blocks": res["greedy_k4"]["blocks"],
-                       "learned_predicted_ms": res["learned"]["predicted_ms"],
-                       "block_mape": res["learned"]["block_mape"],
-                       "gain": res["greedy_k4"]["ms"] / res["learned"]["ms"],
-                       "fidelity_learned_vs_greedy": fid}
-                log(json.dumps(row))
-                rows.append(row)
+        for circ in circs:
+            res = compare(m, circ)
+            row = {"circuit": circ.name, "n": n, "gates": circ.ngates, **{
+                f"{k}_{f}": v[f] for k, v in res.items() for f in ("ms", "launches", "blocks")},
+                "learned_predicted_ms": res["learned"]["predicted_ms"],
+                "learned_launch_mape": res["learned"]["launch_mape"],
+                "learned_plan_ms": res["learned"]["plan_ms"],
+                "gain_vs_greedy": res["greedy_k4"]["ms"] / res["learned"]["ms"],
+                "gain_vs_passes": res["passes_k4"]["ms"] / res["learned"]["ms"],
+                "min_fidelity": min(v["fidelity"] for v in res.values())}
+            log(json.dumps(row))
+            rows.append(row)
     return rows
 
 
@@ -137,8 +156,9 @@ def main(argv=None):
     rows = evaluate(m, ns=(24, 28) if a.quick else (26, 28, 30), log=log)
     report = {"records": int(x.shape[0]), "train_loss_initial": loss0, "train_loss": loss,
               "val_rmse_log": val_rmse, "seconds": time.time() - t0, "heldout": rows,
-              "mean_gain": float(np.mean([r["gain"] for r in rows])),
-              "min_fidelity_learned_vs_greedy": float(min(r["fidelity_learned_vs_greedy"] for r in rows))}
+              "geomean_gain_vs_greedy": float(np.exp(np.mean([np.log(r["gain_vs_greedy"]) for r in rows]))),
+              "geomean_gain_vs_passes": float(np.exp(np.mean([np.log(r["gain_vs_passes"]) for r in rows]))),
+              "min_fidelity": float(min(r["min_fidelity"] for r in rows))}
     if a.report:
         with open(a.report, "w") as f:
             json.dump(report, f, indent=1)

@@ -51,11 +51,16 @@ def test_gradient_matches_finite_differences():
 
 
 def test_block_features():
+    # single-block launch of qubits {3, 10, 20, 29} at tile columns 7 (11 tile bits)
     f = E.block_features(30, 4, 1, 7, [3, 10, 20, 29])
     assert f.shape == (12,)
-    assert f[1] == 1 and f[2] == 0 and f[3] == 1 and f[5] == 1
-    assert abs(f[7] - 0.25) < 1e-7            # one qubit inside the 7-bit run
-    assert abs(f[8] - 3 / 30) < 1e-7 and abs(f[9] - 29 / 30) < 1e-7
-    assert abs(f[10] - 0.25) < 1e-7           # one of the top three qubits
+    assert abs(f[0] - 30 / 34) < 1e-7 and f[1] == 1 and f[2] == 1 / 8
+    assert f[3] == 16 / 128 and f[4] == 1 and abs(f[5] - 1 / 5) < 1e-7
+    # L = {3,10,20,29} + 7 lowest others {0,1,2,4,5,6,7}: contiguous run 0..7 = 8 bits
+    assert abs(f[6] - 8 / 11) < 1e-7
+    assert abs(f[7] - 4 / 14) < 1e-7 and f[8] == 0
+    assert abs(f[9] - 3 / 30) < 1e-7 and abs(f[10] - 29 / 30) < 1e-7
+    assert abs(f[11] - 0.25) < 1e-7           # one of the top three qubits
     f5 = E.block_features(28, 5, 0, 8, [0, 1, 2, 3, 4])
-    assert f5[2] == 1 and f5[3] == 0 and f5[6] == 1 and f5[7] == 1.0
+    assert f5[3] == 32 / 128 and f5[4] == 0 and abs(f5[5] - 3 / 5) < 1e-7
+    assert f5[6] == 1.0 and f5[8] == 1.0 and f5[11] == 0

@@ -135,3 +135,40 @@ def test_fused_large_state_sweep_30q():
         assert abs(sv.norm2() - (2 * 0.707106781186548 ** 2) ** circ.counts().get("H", 0)) < 1e-4
     c = st.classes()["fused"]
     say("fused launches", c["launches"], "ms/launch", c["ms"] / c["launches"], "GB/s", c["gbs"])
+
+
+@pytest.mark.parametrize("group,pass_tb", [(2, 0), (3, 0), (8, 0), (4, 14)])
+@pytest.mark.parametrize("k", [4, 5])
+def test_multi_block_passes_vs_oracle(group, pass_tb, k):
+    n = 18
+    circ = C.random_layers(n, 16, 900 + group + k, haar_frac=0.1)
+    ref, _ = O.run_circuit(circ)
+    say("passes group", group, "tb", pass_tb, "k", k)
+    with E.StateVector(n, E.C64) as sv:
+        st = sv.run(circ, fuse_k=k, split=True, scheduler=2, group=group, pass_tb=pass_tb, profile=True)
+        got = sv.get()
+        single = E.StateVector(n, E.C64)
+        st1 = single.run(circ, fuse_k=k, split=True, profile=True)
+        single.close()
+    fused = st.classes()["fused"]
+    assert fused["launches"] < st1.classes()["fused"]["launches"]     # blocks really shared sweeps
+    fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
+    say("launches", fused["launches"], "vs", st1.classes()["fused"]["launches"], "fidelity", fid)
+    assert fid >= 1 - 1e-6
+    assert np.abs(got - ref.psi).max() < 1e-5
+
+
+def test_passes_with_measurements_keep_rng_accounting():
+    n = 14
+    circ = C.random_layers(n, 6, 5)
+    b = C._Builder(n)
+    for g in circ.gates:
+        b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    b.add(C.MEASURE, 2, 0)
+    for g in C.random_layers(n, 3, 6).gates:
+        b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    c2 = b.build()
+    with E.StateVector(n, E.C64) as f, E.StateVector(n, E.C128) as p:
+        f.run(c2, fuse_k=4, scheduler=2, group=8, rng_state=99)
+        p.run(c2, rng_state=99)
+        assert f.last_rng == p.last_rng
