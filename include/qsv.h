@@ -336,6 +336,25 @@ int qsv_set_costmodel(qsv_state *s, qsv_costmodel *m);
  * QSV_CM_FEATURES) and measured device ms (count); *count = records available */
 int qsv_timing_log(qsv_state *s, float *features, float *ms, int64_t cap, int64_t *count);
 
+/* ---- batched variational circuits (one persistent launch) -------------------
+ * count instances; instance i has nq[i] qubits (2..22) and layers[i] layers of
+ * { RY(theta) then RZ(phi) on every qubit q; CNOT(q, q+1) for q = 0..n-2 } on a
+ * complex64 state from |0...0>.  angles: for each instance, layer, qubit in order,
+ * (theta, phi).  Output zz[i] = sum_{q=0}^{n-2} <Z_q Z_{q+1}> in FP64.
+ * One persistent workgroup per CU takes instances from an atomic work counter
+ * (largest first); instances of <= 14 qubits run entirely in LDS.  No reference
+ * counterpart (QPanda runs one program per process). */
+typedef struct qsv_batch qsv_batch;
+int qsv_batch_create(const int *nq, const int *layers, int64_t count, const double *angles, qsv_batch **out);
+int qsv_batch_destroy(qsv_batch *b);
+/* time each workgroup size (256, 512, 1024 threads) on the batch and keep the fastest */
+int qsv_batch_autotune(qsv_batch *b, int *best_threads, double *best_ms);
+/* one eager launch (threads 0 = tuned); latency_us[i] = device time of instance i */
+int qsv_batch_run(qsv_batch *b, int threads, double *zz, double *latency_us, double *total_ms);
+/* the same launch captured once in a hipGraph and replayed `replays` times;
+ * *total_ms = mean device time per replay, zz = results of the last replay */
+int qsv_batch_graph_run(qsv_batch *b, int replays, double *zz, double *total_ms);
+
 /* ---- Park-Miller "minimal standard" generator (QuantumGates.cpp:46-85) ----- */
 /* Next state from `state` (1..2^31-2); the draw is state / 2147483647.0. */
 int32_t qsv_pm_next(int32_t state);

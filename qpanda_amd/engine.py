@@ -28,7 +28,8 @@ EXPORTED = [
     "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan",
     "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
     "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
-    "qsv_timing_log",
+    "qsv_timing_log", "qsv_batch_create", "qsv_batch_destroy", "qsv_batch_autotune", "qsv_batch_run",
+    "qsv_batch_graph_run",
 ]
 
 CM_FEATURES = 12
@@ -133,6 +134,11 @@ def lib():
             "qsv_cm_train": ([vp, vp, vp, C.c_int64, I, C.c_float, C.POINTER(D)], I),
             "qsv_set_costmodel": ([vp, vp], I),
             "qsv_timing_log": ([vp, vp, vp, C.c_int64, C.POINTER(C.c_int64)], I),
+            "qsv_batch_create": ([vp, vp, C.c_int64, vp, C.POINTER(vp)], I),
+            "qsv_batch_destroy": ([vp], I),
+            "qsv_batch_autotune": ([vp, C.POINTER(I), C.POINTER(D)], I),
+            "qsv_batch_run": ([vp, I, vp, vp, C.POINTER(D)], I),
+            "qsv_batch_graph_run": ([vp, I, vp, C.POINTER(D)], I),
             "qsv_copy_state": ([vp, vp], I),
             "qsv_inner": ([vp, vp, C.POINTER(D), C.POINTER(D)], I),
         }
@@ -405,6 +411,77 @@ class CostModel:
         m = cls(1)
         m.set_params(np.fromfile(path, np.float32))
         return m
+
+
+class Batch:
+    """Many hardware-efficient-ansatz instances evaluated in one persistent launch
+    (see qsv_batch_* in include/qsv.h).  angles: list of (layers, n, 2) arrays."""
+
+    def __init__(self, nq, layers, angles):
+        self.count = len(nq)
+        nq = np.ascontiguousarray(nq, np.int32)
+        ly = np.ascontiguousarray(layers, np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).ravel() for a in angles]))
+        h = C.c_void_p()
+        _check(lib().qsv_batch_create(_fptr(nq), _fptr(ly), self.count, _fptr(flat), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().qsv_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self):
+        t, ms = C.c_int(), C.c_double()
+        _check(lib().qsv_batch_autotune(self.h, C.byref(t), C.byref(ms)))
+        return t.value, ms.value
+
+    def run(self, threads: int = 0):
+        zz = np.zeros(self.count, np.float64)
+        lat = np.zeros(self.count, np.float64)
+        ms = C.c_double()
+        _check(lib().qsv_batch_run(self.h, threads, _fptr(zz), _fptr(lat), C.byref(ms)))
+        return zz, lat, ms.value
+
+    def graph_run(self, replays: int = 3):
+        zz = np.zeros(self.count, np.float64)
+        ms = C.c_double()
+        _check(lib().qsv_batch_graph_run(self.h, replays, _fptr(zz), C.byref(ms)))
+        return zz, ms.value
+
+
+def hea_batch(count: int, seed: int, nmin: int = 10, nmax: int = 22, lmin: int = 1, lmax: int = 8):
+    """Seeded configuration: qubits uniform in [nmin, nmax], layers uniform in
+    [lmin, lmax], angles uniform in [-pi, pi]."""
+    rng = _circ.Xoshiro256(seed)
+    nq, ly, ang = [], [], []
+    for _ in range(count):
+        n = nmin + rng.below(nmax - nmin + 1)
+        L = lmin + rng.below(lmax - lmin + 1)
+        a = np.array([[[np.pi * (2 * rng.uniform() - 1) for _ in range(2)] for _ in range(n)]
+                      for _ in range(L)]).reshape(L, n, 2)
+        nq.append(n)
+        ly.append(L)
+        ang.append(a)
+    return nq, ly, ang
+
+
+def hea_circuit(n: int, angles) -> "_circ.Circuit":
+    """The gate list of one batch instance (RY then RZ per qubit, CNOT ladder)."""
+    b = _circ._Builder(n)
+    for layer in np.asarray(angles).reshape(-1, n, 2):
+        for q in range(n):
+            b.add(_circ.RY, q, theta=float(layer[q, 0]))
+            b.add(_circ.RZ, q, theta=float(layer[q, 1]))
+        for q in range(n - 1):
+            b.add(_circ.CNOT, q, q + 1)
+    return b.build(f"hea_n{n}")
 
 
 def fuse_plan(circ: "_circ.Circuit", max_k: int, exact: bool = False):
