@@ -16,7 +16,11 @@ FP64 comparison; fidelity |<plain|fused>|^2 is computed on the device in FP64.
 
 Roofline: the dominant kernel class of the headline path, timed per launch with
 HIP events on the state's stream inside the timed steps; achieved = algorithmic
-bytes (every amplitude a gate may change, read and written once) / time.
+bytes (every amplitude a gate may change, read and written once) / time.  Fused
+k=4 blocks perform 8 flop per byte moved, below the FP32 MFMA ridge (~20
+flop/B), so they are HBM-bound too; their FP32 MFMA rate is reported beside the
+roofline ("mfma").  "copy_gbs" is a state-to-state copy of the same size timed in
+the same run: the practical streaming rate of the device, for reference.
 
 CPU baseline: QPanda's own CPU simulator (oracle/_ref, OpenMP build) on a bounded
 strided sample of the same circuit at the same size, on this host's cores.
@@ -89,10 +93,10 @@ def measured_traffic(kernel_class, a, headline):
     FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_per_launch.txt, scaled by the copy-kernel
     calibration), when they cover this kernel at this size; else None."""
     key = None
-    if kernel_class == "fused" and a.fuse_k == 4:
-        key = f"k_fused_mfma<4, true>@{a.n}"
+    if kernel_class == "fused":
+        key = f"fused_k{a.fuse_k}_split@{a.n}"
     elif kernel_class == "dense1" and headline == "plain":
-        key = f"k_pair<double2, 2>@{a.n}"
+        key = f"dense1_c128@{a.n}"
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.txt")) as f:
             t = json.load(f)
@@ -134,6 +138,20 @@ def cpu_baseline(circ, seconds, stride):
         del st
         return {"value": done / dt, "unit": "gate-apps/s", "cores": 1, "kind": "port",
                 "sample": f"oracle restatement, 1 thread, n={circ.n}, {done} strided gates in {dt:.1f} s"}
+
+
+def copy_rate(n, reps=5):
+    """State-to-state copy of a complex64 state of n qubits (read + write), GB/s."""
+    from qpanda_amd import engine as E
+    with E.StateVector(n, E.C64) as x, E.StateVector(n, E.C64) as y:
+        y.copy_from(x)
+        y.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            y.copy_from(x)
+        y.sync()
+        dt = (time.perf_counter() - t0) / reps
+    return 2 * 8 * (1 << n) / dt / 1e9
 
 
 def main():
@@ -239,13 +257,13 @@ def main():
     total_bytes = sum(v["bytes"] for v in agg.values())
     total_ms = sum(v["ms"] for v in agg.values())
     per_launch_ms = d["ms"] / d["launches"]
-    if dom == "fused":
-        achieved = d["flops"] / (d["ms"] * 1e9)
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    else:
-        achieved = d["bytes"] / (d["ms"] * 1e6)
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    achieved = d["bytes"] / (d["ms"] * 1e6)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    if dom == "fused":
+        tf = d["flops"] / (d["ms"] * 1e9)
+        result["mfma"] = {"fp32_tflops": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
+                          "flop_per_byte": d["flops"] / d["bytes"]}
     roof["traffic"] = measured_traffic(dom, a, headline)
     roof["kernel_class"] = dom
     roof["avg_launch_ms"] = per_launch_ms
@@ -260,6 +278,8 @@ def main():
                            "hbm_gbs": sum(plain.class_bytes) / (sum(plain.class_ms) * 1e6)}
         result["speedup_vs_plain"] = plain.device_ms / head_stats[-1].device_ms
 
+    if headline == "fused":
+        result["copy_gbs"] = copy_rate(a.n)
     cpu = None
     if not a.no_cpu:
         log("cpu baseline")

@@ -7,25 +7,33 @@
 //     M[2r+1][2c] = Im U[r][c], M[2r+1][2c+1] = Re U[r][c],
 // and X's column j holding group j's amplitudes as (re, im) row pairs.
 //
-// Tiling (one workgroup of 4 waves per tile):
-//   L  = Q u {the 7 lowest qubits not in Q}; L always contains qubits 0..6, so a
-//        tile -- the 2^(k+7) amplitudes whose non-L bits are fixed -- consists of
-//        contiguous runs of >= 128 amplitudes (1 KiB).
-//   load:  16-byte loads of amplitude pairs into LDS, stored as separate re / im
-//          planes indexed by the compact position e over L (conflict-free reads);
+// Tiling:
+//   L  = Q u {the TC lowest qubits not in Q}; L always contains qubits 0..TC-1,
+//        so a tile -- the 2^(k+TC) amplitudes whose non-L bits are fixed --
+//        consists of contiguous runs of >= 2^TC amplitudes (>= 512 B).
+//   One workgroup per tile.  (Persistent workgroups that prefetch the next
+//   tile into registers measured 5-15% slower at 30 qubits: the one-shot
+//   kernel already streams at the in-place read-modify-write rate of a plain
+//   float4 copy kernel, ~5.2 TB/s on MI355X.)
+//   load:  16-byte loads of amplitude pairs (issued before the operand loads),
+//          staged in LDS as separate re / im planes indexed by the compact
+//          position e over L;
 //   math:  wave w owns the 32 groups (columns) w*32..w*32+31 of the tile; per
 //          K-step s it feeds amplitude s of its 32 columns (re in lanes 0-31,
 //          im in lanes 32-63) as the B operand of v_mfma_f32_32x32x2f32, with M's
-//          columns 2s, 2s+1 as A (held in registers for the whole tile);
+//          columns 2s, 2s+1 as A;
 //          split = 1 adds the correction M_lo X with M_lo = M - fp32(M): its
 //          terms are ~2^-24 of the result, so it runs on the BF16 MFMA
-//          (v_mfma_f32_32x32x16_bf16, 16x the FP32 rate) with both operands
-//          rounded to bf16 (relative error of the correction <= 2^-8, i.e.
-//          ~2^-32 of the result), accumulated into the same FP32 registers;
+//          (v_mfma_f32_32x32x16_bf16) with both operands rounded to bf16
+//          (relative error of the correction <= 2^-8, i.e. ~2^-32 of the
+//          result), accumulated into the same FP32 registers.  Its K order is
+//          permuted (K-block sb, lane half hf, element e <-> real column
+//          2(8 sb + e) + hf) so that each lane's B values are exactly the FP32
+//          B values it already holds from the K-steps 8 sb .. 8 sb + 7;
 //   store: results go back to the planes, then 16-byte coalesced stores.
-// Per tile the kernel moves 2 * 8 * 2^(k+7) bytes and performs
-// 2 * (2d)^2 * 128 flops per product, so k = 4 sits near the FP32 MFMA ridge and
-// k = 5 (or k = 4 with the split) is MFMA-bound.
+// Per tile the kernel moves 2 * 8 * 2^(k+TC) bytes and performs
+// 2 * (2d)^2 * 2^TC flops in the FP32 product, so k = 4 is HBM-bound (8 flop/B,
+// below the FP32 MFMA ridge of ~20 flop/B) and k = 5 sits near the ridge.
 //
 // MFMA 32x32x2 f32 operand layout (one block): A[i][k] in lane i + 32k,
 // B[k][j] in lane j + 32k, D[i][j] in lane j + 32*((i/4)%2), register
@@ -59,6 +67,8 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     constexpr int RUN = 1 << TC;             // contiguous low run (qubits 0..TC-1)
     constexpr int HALVES = (2 * D) / 32;     // 32-row output halves: 1 (k=4) or 2 (k=5)
     constexpr int KSTEPS = D;                // K = 2D real rows, 2 per step
+    constexpr int NLD = TSIZE / (2 * THREADS);   // 16-byte loads per thread
+    constexpr int LOBLK = D / 8;             // bf16 K-blocks (16 real rows = 8 amplitudes)
     __shared__ float planes[2][TSIZE];     // [0] = re, [1] = im
     float *const pre = planes[0];
     float *const pim = planes[1];
@@ -68,8 +78,17 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     if (tile >= ntiles) return;
     const uint64_t base = expand_index(tile, fa.tile_ins);
 
+    // issue the tile's loads first; their latency covers the operand loads
+    uint64_t off[NLD];
+    float4 buf[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+        const int e = 2 * (tid + THREADS * i);
+        off[i] = base + (uint64_t)(e & (RUN - 1)) + fa.off_high[e >> TC];
+        buf[i] = *reinterpret_cast<const float4 *>(a + off[i]);
+    }
+
     // A operands (the matrix), operand-ready layout: mops[(h*KSTEPS + s)*64 + lane]
-    constexpr int LOBLK = D / 8;             // bf16 K-blocks (16 real rows = 8 amplitudes)
     float ahi[HALVES][KSTEPS];
     bf16x8 alo[SPLIT ? HALVES : 1][SPLIT ? LOBLK : 1];
 #pragma unroll
@@ -84,14 +103,11 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
             for (int sb = 0; sb < LOBLK; sb++) alo[SPLIT ? h : 0][SPLIT ? sb : 0] = lo[(h * LOBLK + sb) * 64 + lane];
     }
 
-    // load the tile: element pairs (e, e+1) are adjacent in memory (bit 0 is in L)
 #pragma unroll
-    for (int i = 0; i < TSIZE / (2 * THREADS); i++) {
+    for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
-        const uint64_t addr = base + (uint64_t)(e & (RUN - 1)) + fa.off_high[e >> TC];
-        const float4 v = *reinterpret_cast<const float4 *>(a + addr);
-        pre[e] = v.x; pim[e] = v.y;
-        pre[e + 1] = v.z; pim[e + 1] = v.w;
+        pre[e] = buf[i].x; pim[e] = buf[i].y;
+        pre[e + 1] = buf[i].z; pim[e + 1] = buf[i].w;
     }
     __syncthreads();
 
@@ -108,24 +124,19 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     for (int h = 0; h < HALVES; h++)
 #pragma unroll
         for (int v = 0; v < 16; v++) acc[h][v] = 0.0f;
-
+    float bv[KSTEPS];
 #pragma unroll
-    for (int s = 0; s < KSTEPS; s++) {
-        const int e = fa.erow[s] | ecol;
-        const float b = bplane[e];
+    for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[fa.erow[s] | ecol];
 #pragma unroll
-        for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], b, acc[h], 0, 0, 0);
-    }
+    for (int s = 0; s < KSTEPS; s++)
+#pragma unroll
+        for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], bv[s], acc[h], 0, 0, 0);
     if (SPLIT) {
 #pragma unroll
         for (int sb = 0; sb < LOBLK; sb++) {
-            // rows 16sb + 8(lane/32) + e: amplitude 8sb + 4(lane/32) + e/2, part e%2
             bf16x8 bl;
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int idx = fa.erow[8 * sb + 4 * (lane >> 5) + (e >> 1)] | ecol;
-                bl[e] = (__bf16)((e & 1) ? pim[idx] : pre[idx]);
-            }
+            for (int e = 0; e < 8; e++) bl[e] = (__bf16)bv[8 * sb + e];
 #pragma unroll
             for (int h = 0; h < HALVES; h++)
                 acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[SPLIT ? h : 0][SPLIT ? sb : 0], bl, acc[h], 0, 0, 0);
@@ -144,21 +155,20 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
         }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < TSIZE / (2 * THREADS); i++) {
+    for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
-        const uint64_t addr = base + (uint64_t)(e & (RUN - 1)) + fa.off_high[e >> TC];
         float4 v;
         v.x = pre[e]; v.y = pim[e];
         v.z = pre[e + 1]; v.w = pim[e + 1];
-        *reinterpret_cast<float4 *>(a + addr) = v;
+        *reinterpret_cast<float4 *>(a + off[i]) = v;
     }
 }
 
 // Host-side: operand-ready matrices for a block.  u: d x d complex doubles
 // (row bit j = bit(qubits[j])).  Layout: HALVES * D * 64 floats of M_hi
 // (value M[32h + l%32][2s + l/32] at (h*D + s)*64 + l), then, with split,
-// HALVES * (D/8) * 64 * 8 bf16 of M_lo (value M_lo[32h + l%32][16sb + 8(l/32) + e]
-// at ((h*(D/8) + sb)*64 + l)*8 + e).
+// HALVES * (D/8) * 64 * 8 bf16 of M_lo in the permuted K order (value
+// M_lo[32h + l%32][2(8sb + e) + l/32] at ((h*(D/8) + sb)*64 + l)*8 + e).
 static uint16_t to_bf16(float f)
 {
     uint32_t x;
@@ -188,7 +198,7 @@ void fused_operands(int kq, const double *u, int split, float *out)
         for (int sb = 0; sb < lob; sb++)
             for (int l = 0; l < 64; l++)
                 for (int e = 0; e < 8; e++) {
-                    const double v = M(32 * h + (l & 31), 16 * sb + 8 * (l >> 5) + e);
+                    const double v = M(32 * h + (l & 31), 2 * (8 * sb + e) + (l >> 5));
                     lo[((h * lob + sb) * 64 + l) * 8 + e] = to_bf16((float)(v - (double)(float)v));
                 }
 }

@@ -68,21 +68,21 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
         for (int h = 0; h < HALVES; h++)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[h][v] = 0.0f;
+        float bv[KSTEPS];
 #pragma unroll
-        for (int s = 0; s < KSTEPS; s++) {
-            const float b = bplane[pb.erow[s] | ecol];
+        for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[pb.erow[s] | ecol];
 #pragma unroll
-            for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], b, acc[h], 0, 0, 0);
-        }
+        for (int s = 0; s < KSTEPS; s++)
+#pragma unroll
+            for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], bv[s], acc[h], 0, 0, 0);
         if (SPLIT) {
+            // M_lo in the permuted K order of the operand layout: the B values
+            // are this lane's FP32 B values of K-steps 8 sb .. 8 sb + 7
 #pragma unroll
             for (int sb = 0; sb < LOBLK; sb++) {
                 bf16x8 bl;
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int idx = pb.erow[8 * sb + 4 * (lane >> 5) + (e >> 1)] | ecol;
-                    bl[e] = (__bf16)((e & 1) ? pim[idx] : pre[idx]);
-                }
+                for (int e = 0; e < 8; e++) bl[e] = (__bf16)bv[8 * sb + e];
 #pragma unroll
                 for (int h = 0; h < HALVES; h++)
                     acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[SPLIT ? h : 0][SPLIT ? sb : 0], bl, acc[h], 0, 0, 0);
