@@ -9,17 +9,20 @@ gates, synthetic, generated from a fixed seed.
 Paths:
   plain  per-gate complex128 sweeps (one launch per gate), the FP64 reference path;
   fused  complex64 state, gates fused into <= k-qubit blocks run as FP32 MFMA
-         complex GEMMs with the hi+lo split matrix product (when built).
+         complex GEMMs with the hi+lo split matrix product, scheduled by the
+         trained cost model (block size, grouping of blocks into shared HBM
+         sweeps, tile shape); --scheduler greedy gives one sweep per block.
 A step is one full circuit execution from |0...0> on the headline path (fused
 when available, else plain).  The plain path additionally runs once as the
 FP64 comparison; fidelity |<plain|fused>|^2 is computed on the device in FP64.
 
 Roofline: the dominant kernel class of the headline path, timed per launch with
 HIP events on the state's stream inside the timed steps; achieved = algorithmic
-bytes (every amplitude a gate may change, read and written once) / time.  Fused
-k=4 blocks perform 8 flop per byte moved, below the FP32 MFMA ridge (~20
-flop/B), so they are HBM-bound too; their FP32 MFMA rate is reported beside the
-roofline ("mfma").  "copy_gbs" is a state-to-state copy of the same size timed in
+bytes (every amplitude a gate may change, read and written once) / time.  A
+fused k=4 block performs 8 flop per byte moved, below the FP32 MFMA ridge (~20
+flop/B), so a single-block sweep is HBM-bound; a multi-block sweep ("pass")
+performs 8 flop/B per block and is MFMA-bound once its intensity crosses the
+ridge.  The FP32 MFMA rate is always reported ("mfma").  "copy_gbs" is a state-to-state copy of the same size timed in
 the same run: the practical streaming rate of the device, for reference.
 
 CPU baseline: QPanda's own CPU simulator (oracle/_ref, OpenMP build) on a bounded
@@ -51,7 +54,7 @@ def parse():
     p.add_argument("--haar", type=float, default=0.1)
     p.add_argument("--seed", type=int, default=2025)
     p.add_argument("--fuse-k", type=int, default=4)
-    p.add_argument("--scheduler", choices=["greedy", "passes", "learned"], default="greedy",
+    p.add_argument("--scheduler", choices=["greedy", "passes", "learned"], default="learned",
                    help="greedy: one block per sweep; passes: greedy multi-block sweeps; "
                         "learned: cost-model schedule (needs a trained model)")
     p.add_argument("--group", type=int, default=8, help="blocks per sweep for --scheduler passes")
@@ -93,8 +96,8 @@ def measured_traffic(kernel_class, a, headline):
     FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_per_launch.txt, scaled by the copy-kernel
     calibration), when they cover this kernel at this size; else None."""
     key = None
-    if kernel_class == "fused":
-        key = f"fused_k{a.fuse_k}_split@{a.n}"
+    if kernel_class in ("fused", "pass"):
+        key = f"{kernel_class}_k{a.fuse_k}_split@{a.n}"
     elif kernel_class == "dense1" and headline == "plain":
         key = f"dense1_c128@{a.n}"
     try:
@@ -236,7 +239,8 @@ def main():
         if sv_plain is not None:
             result["fidelity_vs_fp64"] = sv_plain.fidelity(sv)
         result["fused"] = {"k": k, "split": True, "scheduler": a.scheduler,
-                           "launches": int(stats[-1].classes().get("fused", {}).get("launches", 0)),
+                           "launches": int(sum(stats[-1].classes().get(c, {}).get("launches", 0)
+                                               for c in ("fused", "pass", "block64"))),
                            "plan_ms": stats[-1].plan_ms,
                            "blocks": int(stats[-1].blocks),
                            "device_ms": stats[-1].device_ms,
@@ -259,11 +263,14 @@ def main():
     per_launch_ms = d["ms"] / d["launches"]
     achieved = d["bytes"] / (d["ms"] * 1e6)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    if dom == "fused":
+    if dom in ("fused", "pass"):
         tf = d["flops"] / (d["ms"] * 1e9)
+        intensity = d["flops"] / d["bytes"]
         result["mfma"] = {"fp32_tflops": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
-                          "flop_per_byte": d["flops"] / d["bytes"]}
+                          "flop_per_byte": intensity}
+        if intensity * HBM_PEAK_GBS / 1e3 >= FP32_PEAK_TFLOPS:      # right of the FP32 MFMA ridge
+            roof = {"bound": "mfma", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = measured_traffic(dom, a, headline)
     roof["kernel_class"] = dom
     roof["avg_launch_ms"] = per_launch_ms

@@ -265,7 +265,7 @@ enum {
     QSV_K_BLOCK64 = 4,  /* dense block in FP64 (two-qubit unitaries, c128 blocks) */
     QSV_K_FUSED = 5,    /* fused FP32 MFMA block (c64)                           */
     QSV_K_READOUT = 6,  /* reductions, collapse                                  */
-    QSV_K_OTHER = 7,
+    QSV_K_PASS = 7,     /* several fused blocks in one sweep (c64)               */
     QSV_K_CLASSES = 8
 };
 

@@ -146,7 +146,7 @@ int record_class(const qsv_state *s, const qsv_gate &r, double *bytes, double *f
 {
     const double S = (double)amp_bytes(s), N = (double)s->dim;
     double touched = N, fl = 0;
-    int cls = QSV_K_OTHER;
+    int cls = QSV_K_DENSE1;              // every kind is listed below (kinds are validated first)
     switch (r.kind) {
     case QSV_GATE_H: case QSV_GATE_RX: case QSV_GATE_RY: case QSV_GATE_U1:
         cls = QSV_K_DENSE1; fl = 14 * N; break;

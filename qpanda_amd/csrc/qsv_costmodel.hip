@@ -273,7 +273,7 @@ int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_ru
                 }
                 if (L.count > 1) {
                     L.tb = pass_tile_bits(q, n);
-                    if (o.pass_tb > L.tb && o.pass_tb <= std::min(14, n)) L.tb = o.pass_tb;
+                    if (o.pass_tb > L.tb && o.pass_tb <= std::min(kMaxPassTileBits, n)) L.tb = o.pass_tb;
                 }
                 launches.push_back(L);
                 b += L.count;
@@ -309,7 +309,7 @@ int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_ru
                 q.insert(q.begin(), &plan[st].qubits);
                 const int tbmin = pass_tile_bits(q, n);
                 if (tbmin < 0) break;
-                for (int tb = tbmin; tb <= std::min(14, n); tb++) {
+                for (int tb = tbmin; tb <= std::min(kMaxPassTileBits, n); tb++) {
                     float f[F];
                     launch_features(n, o.split, q, false, tb, f);
                     cands.push_back({st, g, 0, tb});

@@ -438,9 +438,10 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
             rc = launch_fused_dev(s, it0.qubits.data(), (int)it0.qubits.size(), mp[0], o.split, tc);
         } else {
             rc = launch_pass_dev(s, qsets, mp, o.split, tb);
+            cls = QSV_K_PASS;
         }
         int64_t tag = -1;
-        if (!rc && prof->on() && cls == QSV_K_FUSED) {
+        if (!rc && prof->on() && (cls == QSV_K_FUSED || cls == QSV_K_PASS)) {
             float f[QSV_CM_FEATURES];
             launch_features(s->n, o.split, qsets, single, tb, f);
             tag = (int64_t)(s->tlog_feats.size() / QSV_CM_FEATURES);

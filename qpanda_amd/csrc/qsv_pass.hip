@@ -1,20 +1,24 @@
 // qsv_pass.hip -- several consecutive fused blocks in one HBM sweep.
 //
-// A pass takes G <= 8 consecutive blocks of a fusion plan whose qubit sets
-// together with a low run of qubits fit in TB tile bits (11..14; 2^TB complex64
-// amplitudes = 16..128 KiB of LDS).  L = (union of the blocks' qubits) u
-// {lowest other qubits until |L| = TB}; a tile is the 2^TB amplitudes whose
-// non-L bits are fixed.  One workgroup per tile:
-//   load   16-byte loads into LDS re / im planes (compact position over L);
+// A pass takes G <= 8 consecutive blocks of a fusion plan.  L = qubits 0..3 u
+// (the blocks' qubits) u {lowest other qubits until |L| = TB}, TB = 11..13
+// (2^TB complex64 amplitudes = 16..64 KiB of LDS, so several workgroups share
+// a CU); a tile is the 2^TB amplitudes whose non-L bits are fixed, made of
+// contiguous runs of >= 16 amplitudes (128 B).  One workgroup per tile:
+//   load   16-byte loads issued first; element e of the tile sits at
+//          pdep(e, L), split as pdep(2 tid, L) + pdep(512 i, L) so each thread
+//          computes its address part once;
 //   apply  for each block in order: its 2^(TB-k) groups are the columns; each
 //          wave takes 32-column chunks and runs the same MFMA product as the
 //          single-block kernel (FP32 v_mfma_f32_32x32x2f32 for M_hi, BF16
 //          v_mfma_f32_32x32x16_bf16 for the M_lo correction), writing the
 //          results back in place; a barrier separates blocks;
-//   store  16-byte coalesced stores.
+//   store  16-byte stores to the same addresses.
 // HBM traffic is one read and one write of the state per pass instead of per
-// block; MFMA work is the sum over the blocks.  Whether a grouping pays is a
-// cost-model decision (qsv_costmodel.hip).
+// block; MFMA work is the sum over the blocks.  (A persistent grid-stride form
+// that prefetches the next tile into registers measured ~10% slower at 30
+// qubits than one workgroup per tile.)  Whether a grouping pays is a
+// scheduling decision (greedy grouping, or the cost model in qsv_costmodel.hip).
 #include "qsv_internal.h"
 #include "qsv_sched.h"
 
@@ -33,8 +37,8 @@ struct PassBlock {
 
 struct PassArgs {
     InsertSet tile_ins;     // L positions, ascending
-    int run;                // L contains qubits 0..run-1
-    int hpos[14];           // L positions above the run
+    int lpos[16];           // compact tile bit j -> qubit
+    uint64_t hioff[16];     // pdep(512 i, L): address part of load i (i < 2^(TB-9))
     int nblocks;
     PassBlock b[kMaxPassBlocks];
 };
@@ -100,10 +104,76 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
     }
 }
 
+// k = 4 operands of one block in registers (A of the FP32 product and, with
+// the split, A of the BF16 correction).
+template <bool SPLIT>
+struct Ops4 {
+    float hi[16];
+    bf16x8 lo[SPLIT ? 2 : 1];
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void load_ops4(Ops4<SPLIT> &o, const float *mops, int lane)
+{
+#pragma unroll
+    for (int s = 0; s < 16; s++) o.hi[s] = mops[s * 64 + lane];
+    if (SPLIT) {
+        const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(mops + 16 * 64);
+#pragma unroll
+        for (int sb = 0; sb < 2; sb++) o.lo[SPLIT ? sb : 0] = lo[sb * 64 + lane];
+    }
+}
+
+// One k = 4 block over the tile: 2^(TB-4) columns in 32-column chunks, chunk
+// c on wave c % 4.  (Interleaving two chunks' MFMA chains and prefetching the
+// next block's operands during this one measured within 3% of this form.)
 template <int TB, bool SPLIT>
+__device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBlock &pb, const Ops4<SPLIT> &o,
+                                            int lane, int wave)
+{
+    constexpr int NCB = TB - 4;                      // column bits
+    constexpr int PER = ((1 << NCB) >> 5) / 4;       // chunks per wave: 1, 2, 4
+    const float *const bplane = (lane >> 5) ? pim : pre;
+    for (int c = 0; c < PER; c++) {
+        const int col = (wave + 4 * c) * 32 + (lane & 31);
+        int ecol = 0;
+#pragma unroll
+        for (int i = 0; i < NCB; i++)
+            if (col >> i & 1) ecol |= 1 << pb.col_rank[i];
+        floatx16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+        float bv[16];
+#pragma unroll
+        for (int s = 0; s < 16; s++) bv[s] = bplane[pb.erow[s] | ecol];
+#pragma unroll
+        for (int s = 0; s < 16; s++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(o.hi[s], bv[s], acc, 0, 0, 0);
+        if (SPLIT) {
+#pragma unroll
+            for (int sb = 0; sb < 2; sb++) {
+                bf16x8 bl;
+#pragma unroll
+                for (int e = 0; e < 8; e++) bl[e] = (__bf16)bv[8 * sb + e];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.lo[SPLIT ? sb : 0], bl, acc, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int m = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int e = pb.erow[m >> 1] | ecol;
+            if (m & 1) pim[e] = acc[v];
+            else pre[e] = acc[v];
+        }
+    }
+}
+
+// ANY5: some block of the pass has 5 qubits (otherwise only the k = 4 product
+// is compiled in, which keeps the register count at the k = 4 level).
+template <int TB, bool SPLIT, bool ANY5>
 __global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
 {
     constexpr int TSIZE = 1 << TB;
+    constexpr int NLD = TSIZE / 512;         // 16-byte loads per thread
     __shared__ float planes[2][TSIZE];
     float *const pre = planes[0];
     float *const pim = planes[1];
@@ -111,53 +181,73 @@ __global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint
     const uint64_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     const uint64_t base = expand_index(tile, pa.tile_ins);
-    const int runmask = (1 << pa.run) - 1, nh = TB - pa.run;
-    auto addr_of = [&](int e) -> uint64_t {
-        uint64_t o = (uint64_t)(e & runmask);
-        const int hi = e >> pa.run;
-        for (int j = 0; j < nh; j++)
-            if (hi >> j & 1) o |= uint64_t(1) << pa.hpos[j];
-        return base | o;
-    };
-    for (int i = 0; i < TSIZE / 512; i++) {
+    // element 2 tid + 512 i sits at base + pdep(2 tid, L) + pdep(512 i, L);
+    // bit 0 of L is qubit 0, so element + 1 is the next amplitude in memory
+    uint64_t toff = 0;
+#pragma unroll
+    for (int j = 1; j < 9; j++)
+        if ((tid >> (j - 1)) & 1) toff |= uint64_t(1) << pa.lpos[j];
+    float4 buf[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) buf[i] = *reinterpret_cast<const float4 *>(a + base + toff + pa.hioff[i]);
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + 256 * i);
-        const float4 v = *reinterpret_cast<const float4 *>(a + addr_of(e));
-        pre[e] = v.x; pim[e] = v.y;
-        pre[e + 1] = v.z; pim[e + 1] = v.w;
+        pre[e] = buf[i].x; pim[e] = buf[i].y;
+        pre[e + 1] = buf[i].z; pim[e + 1] = buf[i].w;
     }
     __syncthreads();
-    for (int g = 0; g < pa.nblocks; g++) {
-        if (pa.b[g].kq == 4) pass_apply<4, SPLIT>(pre, pim, pa.b[g], lane, wave);
-        else pass_apply<5, SPLIT>(pre, pim, pa.b[g], lane, wave);
-        __syncthreads();
+    if (ANY5) {
+        for (int g = 0; g < pa.nblocks; g++) {
+            if (pa.b[g].kq == 4) pass_apply<4, SPLIT>(pre, pim, pa.b[g], lane, wave);
+            else pass_apply<5, SPLIT>(pre, pim, pa.b[g], lane, wave);
+            __syncthreads();
+        }
+    } else {
+        for (int g = 0; g < pa.nblocks; g++) {
+            Ops4<SPLIT> ops;
+            load_ops4<SPLIT>(ops, pa.b[g].mops, lane);
+            pass_apply4<TB, SPLIT>(pre, pim, pa.b[g], ops, lane, wave);
+            __syncthreads();
+        }
     }
-    for (int i = 0; i < TSIZE / 512; i++) {
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + 256 * i);
         float4 v;
         v.x = pre[e]; v.y = pim[e];
         v.z = pre[e + 1]; v.w = pim[e + 1];
-        *reinterpret_cast<float4 *>(a + addr_of(e)) = v;
+        *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
     }
 }
 
-// Tile bits a pass over these (widened) blocks needs: the union of their
-// qubits plus at least 5 fill qubits (>= 32 columns for a 5-qubit block and a
-// >= 256 B contiguous run), and at least 11.
+// Tile bits a pass over these (widened) blocks needs: qubits 0..3 (the
+// contiguous run) plus every block qubit above them, at least 11 and at most
+// 13 (64 KiB of LDS).  -1 when they do not fit.
 int pass_tile_bits(const std::vector<const std::vector<int> *> &qsets, int n)
 {
     uint64_t u = 0;
     for (auto *q : qsets)
         for (int x : *q) u |= uint64_t(1) << x;
-    const int tb = std::max(11, __builtin_popcountll(u) + 5);
-    return tb <= std::min(14, n) ? tb : -1;
+    const int tb = std::max(11, 4 + __builtin_popcountll(u & ~uint64_t(15)));
+    return tb <= std::min(kMaxPassTileBits, n) ? tb : -1;
+}
+
+template <int TB, bool ANY5>
+static void launch_tb5(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split)
+{
+    float2 *a = (float2 *)s->amps;
+    if (split) hipLaunchKernelGGL((k_fused_pass<TB, true, ANY5>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, pa);
+    else hipLaunchKernelGGL((k_fused_pass<TB, false, ANY5>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, pa);
 }
 
 template <int TB>
 static void launch_tb(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split)
 {
-    float2 *a = (float2 *)s->amps;
-    if (split) hipLaunchKernelGGL((k_fused_pass<TB, true>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, pa);
-    else hipLaunchKernelGGL((k_fused_pass<TB, false>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, pa);
+    bool any5 = false;
+    for (int g = 0; g < pa.nblocks; g++) any5 |= pa.b[g].kq == 5;
+    if (any5) launch_tb5<TB, true>(s, ntiles, pa, split);
+    else launch_tb5<TB, false>(s, ntiles, pa, split);
 }
 
 // blocks[g]: widened qubit list (4 or 5 qubits) and device operand pointer.
@@ -167,22 +257,30 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused passes need a c64 state");
     const int G = (int)qsets.size();
     if (G < 1 || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..8 blocks per pass");
-    if (tb < 11 || tb > 14 || tb > s->n) return fail(QSV_ERR_ARG, "pass tile bits must be 11..14");
+    if (tb < 11 || tb > kMaxPassTileBits || tb > s->n) return fail(QSV_ERR_ARG, "pass tile bits must be 11..13");
     uint64_t u = 0;
     for (auto *q : qsets) {
         if (q->size() != 4 && q->size() != 5) return fail(QSV_ERR_ARG, "pass blocks must have 4 or 5 qubits");
-        for (int x : *q) u |= uint64_t(1) << x;
+        for (int x : *q) {
+            if (x < 0 || x >= s->n) return fail(QSV_ERR_ARG, "qubit out of range");
+            u |= uint64_t(1) << x;
+        }
     }
-    if (__builtin_popcountll(u) + 5 > tb) return fail(QSV_ERR_ARG, "blocks do not fit the pass tile");
+    if (4 + __builtin_popcountll(u & ~uint64_t(15)) > tb) return fail(QSV_ERR_ARG, "blocks do not fit the pass tile");
     uint64_t lmask = u;
     for (int b = 0; __builtin_popcountll(lmask) < tb; b++) lmask |= uint64_t(1) << b;
     PassArgs pa{};
     pa.tile_ins = insert_set_from_mask(lmask);
     const int *lpos = pa.tile_ins.pos;
-    int run = 0;
-    while (run < tb && lpos[run] == run) run++;
-    pa.run = run;
-    for (int j = 0; j < tb - run; j++) pa.hpos[j] = lpos[run + j];
+    for (int j = 0; j < tb; j++) pa.lpos[j] = lpos[j];
+    if (lpos[0] != 0) return fail(QSV_ERR_ARG, "pass tile without qubit 0");
+    auto pdep = [&](uint64_t e) {
+        uint64_t o = 0;
+        for (int j = 0; j < tb; j++)
+            if (e >> j & 1) o |= uint64_t(1) << lpos[j];
+        return o;
+    };
+    for (int i = 0; i < (1 << (tb - 9)); i++) pa.hioff[i] = pdep((uint64_t)512 * i);
     auto rank = [&](int bit) {
         for (int i = 0; i < tb; i++)
             if (lpos[i] == bit) return i;
@@ -212,8 +310,7 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     switch (tb) {
     case 11: launch_tb<11>(s, ntiles, pa, split); break;
     case 12: launch_tb<12>(s, ntiles, pa, split); break;
-    case 13: launch_tb<13>(s, ntiles, pa, split); break;
-    default: launch_tb<14>(s, ntiles, pa, split); break;
+    default: launch_tb<13>(s, ntiles, pa, split); break;
     }
     s->launches++;
     QSV_HIP(hipGetLastError());

@@ -42,6 +42,7 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
 
 // Multi-block passes (qsv_pass.hip).
 constexpr int kMaxPassBlocks = 8;
+constexpr int kMaxPassTileBits = 13;
 int pass_tile_bits(const std::vector<const std::vector<int> *> &qsets, int n);
 int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
                     const std::vector<const float *> &mops, int split, int tb);

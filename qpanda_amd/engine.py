@@ -53,7 +53,7 @@ class RunStats(C.Structure):
                 ("class_ms", C.c_double * 8), ("class_bytes", C.c_double * 8),
                 ("class_flops", C.c_double * 8), ("plan_ms", C.c_double)]
 
-    CLASS_NAMES = ["dense1", "diag", "perm", "phase", "block64", "fused", "readout", "other"]
+    CLASS_NAMES = ["dense1", "diag", "perm", "phase", "block64", "fused", "readout", "pass"]
 
     def classes(self) -> dict:
         out = {}

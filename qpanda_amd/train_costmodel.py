@@ -7,7 +7,7 @@
    several qubit counts run on the fused path under every decision the
    scheduler can make -- block size k = 2..5, split on / off, single-block
    launches at every tile shape (6, 7, 8 column bits), and multi-block sweeps of
-   up to 2, 3, 4 or 8 blocks at minimal / 13 / 14 tile bits -- with per-launch
+   up to 2, 3, 4 or 8 blocks at minimal / 12 / 13 tile bits -- with per-launch
    event timing; the engine's timing log gives one (features, ms) record per
    launch.
 2. Train: full-batch Adam on the device (the engine's own forward / backward
@@ -33,7 +33,7 @@ from . import engine as E
 SINGLE = [dict(fuse_k=k, split=s, tile=t, scheduler=0) for k in (2, 3, 4, 5) for s in (False, True)
           for t in (6, 7, 8)]
 PASSES = [dict(fuse_k=k, split=s, scheduler=2, group=g, pass_tb=tb) for k in (3, 4, 5) for s in (False, True)
-          for g in (2, 3, 4, 8) for tb in (0, 13, 14)]
+          for g in (2, 3, 4, 8) for tb in (0, 12, 13)]
 
 
 def workloads(ns, seed0, depth):
@@ -98,7 +98,9 @@ def compare(m, circ, reps=3, split=True):
                 st = sv.run(circ, split=split, profile=(rep == reps - 1), **kw)
                 times.append(st.device_ms)
                 plan.append(st.plan_ms)
-            r = {"ms": float(np.median(times)), "launches": int(st.classes().get("fused", {}).get("launches", 0)),
+            cl = st.classes()
+            r = {"ms": float(np.median(times)),
+                 "launches": int(cl.get("fused", {}).get("launches", 0) + cl.get("pass", {}).get("launches", 0)),
                  "blocks": int(st.blocks), "plan_ms": float(np.median(plan)), "fidelity": sv.fidelity(ref)}
             if name == "learned":
                 f, ms = sv.timing_log()

@@ -101,6 +101,11 @@ def test_trained_model_gain_over_greedy():
         rows.append((circ.name, np.median(tg), np.median(tl), fid))
         say(f"{circ.name}: greedy k=4 {np.median(tg):.1f} ms, learned {np.median(tl):.1f} ms, "
             f"gain {np.median(tg) / np.median(tl):.3f}x, fidelity {fid:.12f}")
-    for name, tg, tl, fid in rows:
-        assert tl <= 1.05 * tg, name
+    # the learned schedule groups blocks into shared sweeps where the model
+    # predicts a gain; greedy k = 4 sweeps once per block (held-out circuits:
+    # 1.4-1.8x when the model was trained)
+    gains = [tg / tl for _, tg, tl, _ in rows]
+    for (name, tg, tl, fid), g in zip(rows, gains):
+        assert g >= 1.15, name
         assert fid >= 1 - 1e-6
+    assert np.exp(np.mean(np.log(gains))) >= 1.3

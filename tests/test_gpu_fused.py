@@ -137,7 +137,7 @@ def test_fused_large_state_sweep_30q():
     say("fused launches", c["launches"], "ms/launch", c["ms"] / c["launches"], "GB/s", c["gbs"])
 
 
-@pytest.mark.parametrize("group,pass_tb", [(2, 0), (3, 0), (8, 0), (4, 14)])
+@pytest.mark.parametrize("group,pass_tb", [(2, 0), (3, 0), (8, 0), (4, 13)])
 @pytest.mark.parametrize("k", [4, 5])
 def test_multi_block_passes_vs_oracle(group, pass_tb, k):
     n = 18
@@ -150,10 +150,12 @@ def test_multi_block_passes_vs_oracle(group, pass_tb, k):
         single = E.StateVector(n, E.C64)
         st1 = single.run(circ, fuse_k=k, split=True, profile=True)
         single.close()
-    fused = st.classes()["fused"]
-    assert fused["launches"] < st1.classes()["fused"]["launches"]     # blocks really shared sweeps
+    cl = st.classes()
+    sweeps = cl.get("fused", {}).get("launches", 0) + cl.get("pass", {}).get("launches", 0)
+    assert cl.get("pass", {}).get("launches", 0) > 0
+    assert sweeps < st1.classes()["fused"]["launches"]     # blocks really shared sweeps
     fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
-    say("launches", fused["launches"], "vs", st1.classes()["fused"]["launches"], "fidelity", fid)
+    say("sweeps", sweeps, "vs", st1.classes()["fused"]["launches"], "fidelity", fid)
     assert fid >= 1 - 1e-6
     assert np.abs(got - ref.psi).max() < 1e-5
 
