@@ -1,10 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark: gate-applications/s and achieved HBM bandwidth at 30 qubits.
 
-Workload: a 30-qubit random circuit of depth 200 (per layer: H/RX/RY/RZ on every
-qubit with angles uniform in [0, 2*pi); a seeded perfect pairing with CNOT or CZ,
-10% of the pairs replaced by Haar-random two-qubit unitaries), 9,000 logical
-gates, synthetic, generated from a fixed seed.
+Default workload (--workload random30): a 30-qubit random circuit of depth 200
+(per layer: H/RX/RY/RZ on every qubit with angles uniform in [0, 2*pi); a seeded
+perfect pairing with CNOT or CZ, 10% of the pairs replaced by Haar-random
+two-qubit unitaries), 9,000 logical gates, synthetic, generated from a fixed seed.
 
 Paths:
   plain  per-gate complex128 sweeps (one launch per gate), the FP64 reference path;
@@ -12,9 +12,19 @@ Paths:
          complex GEMMs with the hi+lo split matrix product, scheduled by the
          trained cost model (block size, grouping of blocks into shared HBM
          sweeps, tile shape); --scheduler greedy gives one sweep per block.
-A step is one full circuit execution from |0...0> on the headline path (fused
-when available, else plain).  The plain path additionally runs once as the
-FP64 comparison; fidelity |<plain|fused>|^2 is computed on the device in FP64.
+A step is one full circuit execution from |0...0> on the headline path (fused).
+The plain path and the greedy k=4 fused path each run once for comparison;
+fidelity |<plain|fused>|^2 is computed on the device in FP64.
+
+Other workloads (one JSON line each; not the default line):
+  --workload large33  33-qubit random circuit of depth 60, complex64 (64 GiB),
+                      learned schedule, FP64 norm, 10^5 measurement shots from
+                      the final state (linear cross-entropy of the samples), and
+                      the fidelity of the same pipeline against the FP64 path on
+                      the 28-qubit restriction of the same seeded generator;
+  --workload batch    4096 hardware-efficient-ansatz instances (10..22 qubits,
+                      1..8 layers) in one persistent launch under hipGraph
+                      replay; instances/s with per-instance p50 / p99 latency.
 
 Roofline: the dominant kernel class of the headline path, timed per launch with
 HIP events on the state's stream inside the timed steps; achieved = algorithmic
@@ -22,8 +32,9 @@ bytes (every amplitude a gate may change, read and written once) / time.  A
 fused k=4 block performs 8 flop per byte moved, below the FP32 MFMA ridge (~20
 flop/B), so a single-block sweep is HBM-bound; a multi-block sweep ("pass")
 performs 8 flop/B per block and is MFMA-bound once its intensity crosses the
-ridge.  The FP32 MFMA rate is always reported ("mfma").  "copy_gbs" is a state-to-state copy of the same size timed in
-the same run: the practical streaming rate of the device, for reference.
+ridge.  The FP32 MFMA rate is always reported ("mfma").  "copy_gbs" is a
+state-to-state copy of the same size timed in the same run: the practical
+streaming rate of the device, for reference.
 
 CPU baseline: QPanda's own CPU simulator (oracle/_ref, OpenMP build) on a bounded
 strided sample of the same circuit at the same size, on this host's cores.
@@ -37,6 +48,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -49,24 +62,32 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--n", type=int, default=30)
-    p.add_argument("--depth", type=int, default=200)
+    p.add_argument("--workload", choices=["random30", "large33", "batch"], default="random30")
+    p.add_argument("--n", type=int, default=None, help="qubits (random30: 30, large33: 33)")
+    p.add_argument("--depth", type=int, default=None, help="layers (random30: 200, large33: 60)")
+    p.add_argument("--shots", type=int, default=100000, help="large33: measurement shots")
+    p.add_argument("--count", type=int, default=4096, help="batch: instances")
     p.add_argument("--haar", type=float, default=0.1)
     p.add_argument("--seed", type=int, default=2025)
-    p.add_argument("--fuse-k", type=int, default=4)
+    p.add_argument("--fuse-k", type=int, default=4, help="block size for greedy / passes (learned picks its own)")
     p.add_argument("--scheduler", choices=["greedy", "passes", "learned"], default="learned",
                    help="greedy: one block per sweep; passes: greedy multi-block sweeps; "
                         "learned: cost-model schedule (needs a trained model)")
     p.add_argument("--group", type=int, default=8, help="blocks per sweep for --scheduler passes")
     p.add_argument("--costmodel", default=None, help="trained cost-model parameters (float32)")
-    p.add_argument("--path", choices=["auto", "plain", "fused"], default="auto")
-    p.add_argument("--no-plain", action="store_true", help="skip the FP64 comparison run")
+    p.add_argument("--path", choices=["fused", "plain"], default="fused")
+    p.add_argument("--no-plain", action="store_true", help="skip the FP64 and greedy comparison runs")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--cpu-stride", type=int, default=97)
     p.add_argument("--probe", action="store_true",
                    help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
-    return p.parse_args()
+    a = p.parse_args()
+    if a.n is None:
+        a.n = 33 if a.workload == "large33" else 30
+    if a.depth is None:
+        a.depth = 60 if a.workload == "large33" else 200
+    return a
 
 
 def probe(a):
@@ -91,13 +112,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel_class, a, headline):
+def measured_traffic(kernel_class, a, headline, k):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_per_launch.txt, scaled by the copy-kernel
     calibration), when they cover this kernel at this size; else None."""
     key = None
     if kernel_class in ("fused", "pass"):
-        key = f"{kernel_class}_k{a.fuse_k}_split@{a.n}"
+        key = f"{kernel_class}_k{k}_split@{a.n}"
     elif kernel_class == "dense1" and headline == "plain":
         key = f"dense1_c128@{a.n}"
     try:
@@ -157,6 +178,50 @@ def copy_rate(n, reps=5):
     return 2 * 8 * (1 << n) / dt / 1e9
 
 
+def load_costmodel(a):
+    from qpanda_amd import engine as E
+    path = a.costmodel or E.DEFAULT_COSTMODEL
+    if not os.path.exists(path):
+        raise SystemExit(f"--scheduler learned needs a trained cost model ({path})")
+    return E.CostModel.load(path)
+
+
+def roofline_of(stats_list, a, headline, result):
+    """Dominant kernel class over the given runs: roofline object, MFMA rate."""
+    agg = {}
+    for st in stats_list:
+        for name, c in st.classes().items():
+            d = agg.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            for key in d:
+                d[key] += c[key]
+    dom = max(agg, key=lambda nm: agg[nm]["ms"])
+    d = agg[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e6)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    if dom in ("fused", "pass"):
+        tf = d["flops"] / (d["ms"] * 1e9)
+        intensity = d["flops"] / d["bytes"]
+        result["mfma"] = {"fp32_tflops": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
+                          "flop_per_byte": intensity}
+        if intensity * HBM_PEAK_GBS / 1e3 >= FP32_PEAK_TFLOPS:      # right of the FP32 MFMA ridge
+            roof = {"bound": "mfma", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    k_used = int(stats_list[-1].fuse_k) or a.fuse_k
+    roof["traffic"] = measured_traffic(dom, a, headline, k_used)
+    roof["kernel_class"] = dom
+    roof["avg_launch_ms"] = d["ms"] / d["launches"]
+    roof["algorithmic_bytes_per_launch"] = d["bytes"] / d["launches"]
+    total_bytes = sum(v["bytes"] for v in agg.values())
+    total_ms = sum(v["ms"] for v in agg.values())
+    result["hbm_gbs"] = total_bytes / (total_ms * 1e6)
+    result["hbm_frac"] = result["hbm_gbs"] / HBM_PEAK_GBS
+    n_runs = max(1, len(stats_list))
+    result["kernel_classes"] = {k2: {"launches": int(v["launches"] / n_runs), "ms": v["ms"] / n_runs,
+                                     "GBs": v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0.0}
+                                for k2, v in agg.items()}
+    return roof
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -166,35 +231,24 @@ def main():
     if a.probe:
         probe(a)
         return
+    if a.workload == "large33":
+        return main_large(a)
+    if a.workload == "batch":
+        return main_batch(a)
     from qpanda_amd import circuits as C
     from qpanda_amd import engine as E
 
     circ = C.random_layers(a.n, a.depth, a.seed, haar_frac=a.haar)
     G = circ.ngates
     log(f"workload n={a.n} depth={a.depth} gates={G} counts={circ.counts()}")
-
-    fused_ok = a.path in ("auto", "fused")
-    if fused_ok:
-        try:
-            with E.StateVector(12, E.C64) as chk:
-                chk.run(C.ghz(12), fuse_k=4)
-        except E.QsvError as e:
-            if a.path == "fused":
-                raise
-            log("fused path unavailable:", e)
-            fused_ok = False
-    headline = "fused" if fused_ok else "plain"
+    headline = a.path
     steps = a.steps if a.steps is not None else (3 if headline == "fused" else 1)
     warmup = a.warmup if a.warmup is not None else (1 if headline == "fused" else 0)
     sched = {"greedy": 0, "learned": 1, "passes": 2}[a.scheduler]
-    cm = None
-    if sched == 1:
-        path = a.costmodel or E.DEFAULT_COSTMODEL
-        if not os.path.exists(path):
-            raise SystemExit(f"--scheduler learned needs a trained cost model ({path})")
-        cm = E.CostModel.load(path)
+    cm = load_costmodel(a) if sched == 1 else None
 
     plain = None
+    greedy = None
     result = {}
     sv_plain = None
     if headline == "plain" or not a.no_plain:
@@ -224,6 +278,11 @@ def main():
         if cm is not None:
             sv.attach_costmodel(cm)
         k = a.fuse_k
+        if sv_plain is not None and sched != 0:
+            log("greedy k=4 fused path: one run (comparison)")
+            greedy = sv.run(circ, fuse_k=4, split=True, scheduler=0, profile=True)
+            result["greedy_k4"] = {"gate_apps_per_s": G / (greedy.device_ms / 1e3), "device_ms": greedy.device_ms,
+                                   "sweeps": int(greedy.launches), "fidelity_vs_fp64": sv_plain.fidelity(sv)}
         for _ in range(warmup):
             sv.reset()
             sv.run(circ, fuse_k=k, split=True, scheduler=sched, group=a.group)
@@ -238,9 +297,9 @@ def main():
         head_stats = stats
         if sv_plain is not None:
             result["fidelity_vs_fp64"] = sv_plain.fidelity(sv)
-        result["fused"] = {"k": k, "split": True, "scheduler": a.scheduler,
-                           "launches": int(sum(stats[-1].classes().get(c, {}).get("launches", 0)
-                                               for c in ("fused", "pass", "block64"))),
+        cl = stats[-1].classes()
+        result["fused"] = {"k": int(stats[-1].fuse_k), "split": True, "scheduler": a.scheduler,
+                           "sweeps": int(sum(cl.get(c, {}).get("launches", 0) for c in ("fused", "pass", "block64"))),
                            "plan_ms": stats[-1].plan_ms,
                            "blocks": int(stats[-1].blocks),
                            "device_ms": stats[-1].device_ms,
@@ -249,41 +308,13 @@ def main():
 
     ms_per_step = wall * 1e3 / steps
     value = G / (wall / steps)
-    # dominant kernel class over the timed steps
-    agg = {}
-    for st in head_stats:
-        for name, c in st.classes().items():
-            d = agg.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
-            for key in d:
-                d[key] += c[key]
-    dom = max(agg, key=lambda nm: agg[nm]["ms"])
-    d = agg[dom]
-    total_bytes = sum(v["bytes"] for v in agg.values())
-    total_ms = sum(v["ms"] for v in agg.values())
-    per_launch_ms = d["ms"] / d["launches"]
-    achieved = d["bytes"] / (d["ms"] * 1e6)
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-    if dom in ("fused", "pass"):
-        tf = d["flops"] / (d["ms"] * 1e9)
-        intensity = d["flops"] / d["bytes"]
-        result["mfma"] = {"fp32_tflops": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
-                          "flop_per_byte": intensity}
-        if intensity * HBM_PEAK_GBS / 1e3 >= FP32_PEAK_TFLOPS:      # right of the FP32 MFMA ridge
-            roof = {"bound": "mfma", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic(dom, a, headline)
-    roof["kernel_class"] = dom
-    roof["avg_launch_ms"] = per_launch_ms
-    roof["algorithmic_bytes_per_launch"] = d["bytes"] / d["launches"]
-    result["hbm_gbs"] = total_bytes / (total_ms * 1e6)
-    result["hbm_frac"] = result["hbm_gbs"] / HBM_PEAK_GBS
-    result["kernel_classes"] = {k2: {"launches": int(v["launches"] / steps), "ms": v["ms"] / steps,
-                                     "GBs": v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0.0}
-                                for k2, v in agg.items()}
+    roof = roofline_of(head_stats, a, headline, result)
     if plain is not None:
         result["plain"] = {"gate_apps_per_s": G / (plain.device_ms / 1e3), "device_ms": plain.device_ms,
                            "hbm_gbs": sum(plain.class_bytes) / (sum(plain.class_ms) * 1e6)}
         result["speedup_vs_plain"] = plain.device_ms / head_stats[-1].device_ms
+    if greedy is not None:
+        result["speedup_vs_greedy_k4"] = greedy.device_ms / head_stats[-1].device_ms
 
     if headline == "fused":
         result["copy_gbs"] = copy_rate(a.n)
@@ -315,6 +346,148 @@ def main():
     }
     line.update(result)
     print(json.dumps(line), flush=True)
+
+
+def main_large(a):
+    """33 qubits, complex64: learned schedule, FP64 norm, shots, 28-qubit fidelity."""
+    from qpanda_amd import circuits as C
+    from qpanda_amd import engine as E
+    circ = C.random_layers(a.n, a.depth, a.seed, haar_frac=a.haar)
+    G = circ.ngates
+    log(f"workload n={a.n} depth={a.depth} gates={G}")
+    cm = load_costmodel(a)
+    steps = a.steps if a.steps is not None else 1
+    warmup = a.warmup if a.warmup is not None else 0
+    result = {}
+    with E.StateVector(a.n, E.C64) as sv:
+        sv.attach_costmodel(cm)
+        for _ in range(warmup):
+            sv.reset()
+            sv.run(circ, fuse_k=5, split=True, scheduler=1)
+        sv.sync()
+        stats = []
+        t0 = time.perf_counter()
+        for i in range(steps):
+            sv.reset()
+            stats.append(sv.run(circ, fuse_k=5, split=True, scheduler=1, profile=True))
+            log(f"step {i}: device {stats[-1].device_ms:.0f} ms")
+        sv.sync()
+        wall = time.perf_counter() - t0
+        st = stats[-1]
+        norm = sv.norm2()
+        result["norm2_fp64"] = norm
+        cl = st.classes()
+        result["fused"] = {"k": int(st.fuse_k), "split": True, "scheduler": "learned", "blocks": int(st.blocks),
+                           "sweeps": int(sum(cl.get(c, {}).get("launches", 0) for c in ("fused", "pass", "block64"))),
+                           "plan_ms": st.plan_ms, "device_ms": st.device_ms, "predicted_ms": st.model_ms_pred}
+        # shots: all qubits in order from the final state, one draw per gate skipped per shot
+        log(f"{a.shots} shots")
+        t1 = time.perf_counter()
+        words, _, ties = sv.sample_shots(list(range(a.n)), G, 20250101, a.shots)
+        t_shots = time.perf_counter() - t1
+        amps = sv.gather(words)
+        p = np.abs(amps) ** 2 / norm
+        result["shots"] = {"count": a.shots, "seconds": t_shots, "shots_per_s": a.shots / t_shots,
+                           "fallback_shots": int(ties), "distinct": int(np.unique(words).size),
+                           "linear_xeb": float((1 << a.n) * p.mean() - 1)}
+    roof = roofline_of(stats, a, "fused", result)
+    # fidelity of the same pipeline on the 28-qubit restriction of the generator
+    nr = 28
+    c28 = C.random_layers(nr, a.depth, a.seed, haar_frac=a.haar)
+    log("28-qubit restriction: learned c64 vs plain c128")
+    with E.StateVector(nr, E.C64) as f, E.StateVector(nr, E.C128) as pl:
+        f.attach_costmodel(cm)
+        f.run(c28, fuse_k=5, split=True, scheduler=1)
+        pl.run(c28)
+        result["fidelity_vs_fp64_28q"] = pl.fidelity(f)
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline (28-qubit restriction; the 33-qubit c128 state needs 128 GiB of host memory)")
+        cpu = cpu_baseline(c28, a.cpu_seconds, a.cpu_stride)
+    line = {
+        "metric": f"gate-apps/s at {a.n}q (random circuit, depth {a.depth}, complex64)",
+        "value": G / (wall / steps), "unit": "gate-apps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 mfma, fp64 reductions)", "data": "synthetic seeded circuit",
+        "config": {"workload": f"random circuit n={a.n} depth={a.depth} haar_frac={a.haar} seed={a.seed} "
+                               f"({G} gates), {a.shots} shots", "n_qubits": a.n, "depth": a.depth, "gates": G},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+def main_batch(a):
+    """4096 hardware-efficient-ansatz instances in one persistent launch."""
+    from qpanda_amd import engine as E
+    nq, ly, ang = E.hea_batch(a.count, a.seed)
+    steps = a.steps if a.steps is not None else 5
+    warmup = a.warmup if a.warmup is not None else 1
+    b = E.Batch(nq, ly, ang)
+    best, _ = b.autotune()
+    for _ in range(warmup):
+        b.run()
+    zz, lat, eager_ms = b.run()
+    t0 = time.perf_counter()
+    zg, graph_ms = b.graph_run(replays=steps)
+    wall = time.perf_counter() - t0
+    b.close()
+    nq = np.asarray(nq)
+    ly = np.asarray(ly)
+    # algorithmic HBM bytes: instances above 14 qubits stream their state twice per layer
+    big = nq > 14
+    hbm_bytes = float(np.sum(ly[big] * 2 * 2 * 8 * (2.0 ** nq[big])))
+    gate_apps = float(np.sum(ly * (2 * nq - 1)))
+    result = {
+        "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "eager_ms": eager_ms, "graph_replay_ms": graph_ms, "threads": best,
+        "graph_equals_eager": bool(np.array_equal(zg, zz)),
+        "gate_apps_per_s": gate_apps / (graph_ms / 1e3),
+        "hbm_gbs_streamed_instances": hbm_bytes / (graph_ms * 1e6),
+    }
+    roof = {"bound": "hbm", "achieved": result["hbm_gbs_streamed_instances"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": result["hbm_gbs_streamed_instances"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel_class": "batch",
+            "note": "instances of <= 14 qubits stay in LDS and move no HBM bytes; bytes count the two "
+                    "LDS-tiled passes per layer of the larger ones"}
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline_batch(nq, ang, a.cpu_seconds)
+    line = {
+        "metric": f"instances/s ({a.count} HEA instances, 10-22 qubits, <Z_i Z_i+1> sums)",
+        "value": a.count / (graph_ms / 1e3), "unit": "instances/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": graph_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 state, fp64 reductions)", "data": "synthetic seeded instances",
+        "config": {"workload": f"{a.count} hardware-efficient-ansatz instances seed={a.seed}",
+                   "instances": a.count}, "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_batch(nq, ang, seconds):
+    """QPanda's CPU simulator on instances 0, 1, 2, ... until `seconds` elapse."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from qpanda_amd import engine as E
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    if not O.ref_available():
+        return None
+    done, spent = 0, 0.0
+    with tempfile.TemporaryDirectory() as td:
+        for n, an in zip(nq, ang):
+            path = os.path.join(td, "inst.qsvc")
+            E.hea_circuit(int(n), an).save(path)
+            r = O.ref_time(path, 1, 1e9, omp=True, threads=threads)
+            spent += r["seconds"]
+            done += 1
+            if spent > seconds:
+                break
+    return {"value": done / spent, "unit": "instances/s", "cores": threads, "kind": "reference",
+            "sample": f"QPanda CPUQuantumGates (OpenMP build, OMP_NUM_THREADS={threads}), c128, the first "
+                      f"{done} instances of the batch one after another in {spent:.1f} s (allocation excluded)"}
 
 
 if __name__ == "__main__":

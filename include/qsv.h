@@ -118,6 +118,11 @@ int qsv_precision(const qsv_state *s);
 int qsv_set_state(qsv_state *s, const double *amps);
 int qsv_get_state(qsv_state *s, double *amps);
 
+/* amps[2i], amps[2i+1] = amplitude idx[i] (FP64; indices < 2^n, checked).
+ * No reference counterpart: reads single amplitudes of a state too large to
+ * copy out, e.g. the probabilities of sampled outcomes. */
+int qsv_gather(qsv_state *s, const uint64_t *idx, int64_t count, double *amps);
+
 /* Device-side copy of src into dst (same qubit count; precisions may differ,
  * converting c128 <-> c64).  A streaming kernel, used for snapshots and for
  * comparing paths.  No reference counterpart. */
@@ -226,12 +231,25 @@ int qsv_expect_zz_chain(qsv_state *s, double *value);
  * shot from |0...0>, runs the program (one draw per gate call) and then measures;
  * because the gates do not depend on the draws, the pre-measurement state is the
  * same every shot.  With that state in `s` (left unchanged), each shot skips
- * `gate_draws` draws by jump-ahead and measures qubits[0..m-1] in order with the
- * reference's decision rule; bit j of out[shot] is the outcome of qubits[j].
- * *rng is the generator state before the first shot's gates and is advanced past
- * the last shot.  *ties (may be NULL) counts decisions re-made in serial order. */
+ * `gate_draws` draws by jump-ahead and measures qubits[0..m-1] (distinct) in
+ * order with the reference's decision rule (CPUQuantumGates.cpp:1071-1126);
+ * bit j of out[shot] is the outcome of qubits[j].  *rng is the generator state
+ * before the first shot's gates and is advanced past the last shot.
+ * No state copy per shot: the decisions are evaluated from FP64 prefix sums of
+ * |a|^2 (a marginal tree plus per-shot LDS reductions); a shot with a decision
+ * inside the measurement tie band is re-run by the copy path below, so the
+ * outcomes equal qsv_sample_shots_copy's (for a complex64 state: on its FP64
+ * conversion).  *ties (may be NULL) counts shots re-run that way plus serial
+ * re-decisions inside them. */
 int qsv_sample_shots(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
                      int64_t shots, uint64_t *out, int64_t *ties);
+
+/* The same shot loop shaped as the reference's: per shot, copy the state and
+ * measure it qubit by qubit (collapse and renormalise each time).  Cost: one
+ * state copy plus 2m sweeps per shot; kept as the cross-check of
+ * qsv_sample_shots.  *ties counts serial re-decisions. */
+int qsv_sample_shots_copy(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
+                          int64_t shots, uint64_t *out, int64_t *ties);
 
 /* ---- whole circuits ---------------------------------------------------------
  * Apply gates[0..ngates-1] in order.  mats = matrix pool (complex doubles) for
@@ -285,6 +303,8 @@ typedef struct qsv_run_stats {
     double  class_flops[QSV_K_CLASSES];
     double  plan_ms;        /* host time spent planning (fused path); the   */
                             /* device time above starts after planning      */
+    int64_t fuse_k;         /* block size the fused plan used (the learned  */
+                            /* scheduler picks it), 0 on the per-gate path  */
 } qsv_run_stats;
 
 /* Greedy fusion plan of a gate list (host only, no device needed): blocks of at

@@ -250,8 +250,9 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
     QSV_HIP(hipEventRecord(e0, s->stream));
     int64_t blocks = 0;
     double pred_ms = 0, plan_ms = 0;
+    int k_used = 0;
     if (o.fuse_k >= 2) {
-        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof, e0, &plan_ms);
+        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof, e0, &plan_ms, &k_used);
     } else {
         for (int64_t i = 0; i < ngates && !rc; i++) {
             rc = prof.begin();
@@ -282,6 +283,7 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
         stats->device_ms = ms;
         stats->model_ms_pred = pred_ms;
         stats->plan_ms = plan_ms;
+        stats->fuse_k = k_used;
         if (!rc) rc = prof.collect(stats);
     }
     if (!rc && prof.on() && o.fuse_k >= 2) {

@@ -177,6 +177,16 @@ int qsv_get_state(qsv_state *s, double *amps)
     return QSV_OK;
 }
 
+int qsv_gather(qsv_state *s, const uint64_t *idx, int64_t count, double *amps)
+{
+    CHECK_STATE(s);
+    if (count < 0 || (count && (!idx || !amps))) return fail(QSV_ERR_ARG, "bad gather arguments");
+    for (int64_t i = 0; i < count; i++)
+        if (idx[i] >= s->dim) return fail(QSV_ERR_ARG, "gather index out of range");
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    return gather(s, idx, count, amps);
+}
+
 int qsv_copy_state(qsv_state *dst, qsv_state *src)
 {
     CHECK_STATE(dst);
@@ -423,42 +433,39 @@ int qsv_measure(qsv_state *s, int q, double r, int *outcome)
     return QSV_OK;
 }
 
+static int check_sample_args(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
+                             int64_t shots, uint64_t *out)
+{
+    if (!qubits || !rng || (!out && shots) || shots < 0 || gate_draws < 0) return fail(QSV_ERR_ARG, "bad argument");
+    if (m < 1 || m > 64 || m > s->n) return fail(QSV_ERR_ARG, "1..min(64, n) measured qubits");
+    uint64_t seen = 0;
+    for (int j = 0; j < m; j++) {
+        int rc = check_qubit(s, qubits[j]);
+        if (rc) return rc;
+        if (seen >> qubits[j] & 1) return fail(QSV_ERR_QUBIT, "repeated qubit");
+        seen |= uint64_t(1) << qubits[j];
+    }
+    return QSV_OK;
+}
+
 int qsv_sample_shots(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
                      int64_t shots, uint64_t *out, int64_t *ties)
 {
     CHECK_STATE(s);
-    if (!qubits || !rng || !out || shots < 0 || gate_draws < 0) return fail(QSV_ERR_ARG, "bad argument");
-    if (m < 1 || m > 64) return fail(QSV_ERR_ARG, "1..64 measured qubits");
-    for (int j = 0; j < m; j++) {
-        int rc = check_qubit(s, qubits[j]);
-        if (rc) return rc;
-    }
-    void *work = nullptr;
-    const size_t bytes = s->dim * amp_bytes(s);
-    if (hipMalloc(&work, bytes) != hipSuccess) return fail(QSV_ERR_ALLOC, "cannot allocate shot scratch");
-    qsv_state w = *s;                 // same stream and scratch, different amplitudes
-    w.amps = work;
-    int rc = QSV_OK;
-    const int64_t ties0 = s->tie_resolutions;
-    for (int64_t sh = 0; sh < shots && !rc; sh++) {
-        hipError_t e = hipMemcpyAsync(work, s->amps, bytes, hipMemcpyDeviceToDevice, s->stream);
-        if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpyAsync"); break; }
-        *rng = qsv_pm_jump(*rng, (uint64_t)gate_draws);
-        uint64_t bits = 0;
-        for (int j = 0; j < m && !rc; j++) {
-            *rng = qsv_pm_next(*rng);
-            int o = 0;
-            rc = qsv_measure(&w, qubits[j], (double)*rng / 2147483647, &o);
-            bits |= (uint64_t)o << j;
-        }
-        out[sh] = bits;
-    }
-    s->launches = w.launches;
-    s->tie_resolutions = w.tie_resolutions;
-    if (ties) *ties = s->tie_resolutions - ties0;
-    (void)hipStreamSynchronize(s->stream);
-    (void)hipFree(work);
-    return rc;
+    int rc = check_sample_args(s, qubits, m, gate_draws, rng, shots, out);
+    if (rc) return rc;
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    return sample_tree(s, qubits, m, gate_draws, rng, shots, out, ties);
+}
+
+int qsv_sample_shots_copy(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng,
+                          int64_t shots, uint64_t *out, int64_t *ties)
+{
+    CHECK_STATE(s);
+    int rc = check_sample_args(s, qubits, m, gate_draws, rng, shots, out);
+    if (rc) return rc;
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    return sample_copy(s, qubits, m, gate_draws, rng, shots, out, ties);
 }
 
 int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out)

@@ -325,7 +325,7 @@ void widen_to(PlanItem &it, int kq)
 
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &o, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms,
-              Profiler *prof, hipEvent_t start, double *plan_ms)
+              Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used)
 {
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "the fused path runs on c64 states");
     const auto t_plan = std::chrono::steady_clock::now();
@@ -385,6 +385,7 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
     const double N = (double)s->dim;
     int64_t nblocks = 0;
     *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
+    *k_used = choice.fuse_k;
     QSV_HIP(hipEventRecord(start, s->stream));       // device time starts after planning
     rc = QSV_OK;
     if (prof->on()) {

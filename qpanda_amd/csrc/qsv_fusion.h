@@ -32,9 +32,11 @@ private:
     std::vector<float> tag_ms_;
 };
 
-// Fuse `gates` into blocks of at most opts.fuse_k qubits and apply them.
+// Fuse `gates` into blocks of at most opts.fuse_k qubits (the learned
+// scheduler picks the block size itself) and apply them; *k_used = the block
+// size of the plan.
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &opts, int32_t *rng, int *outcomes, int64_t *blocks,
-              double *pred_ms, Profiler *prof, hipEvent_t start, double *plan_ms);
+              double *pred_ms, Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used);
 
 }  // namespace qsv

@@ -101,6 +101,11 @@ int reduce_zz_chain(qsv_state *s, double *v);
 int serial_prob0(qsv_state *s, int q, double *p0);
 int inner(qsv_state *a, qsv_state *b, double *re, double *im);
 int copy_state(qsv_state *d, qsv_state *s);
+int sample_tree(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng, int64_t shots,
+                uint64_t *out, int64_t *ties);
+int sample_copy(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng, int64_t shots,
+                uint64_t *out, int64_t *ties);
+int gather(qsv_state *s, const uint64_t *idx, int64_t count, double *out);
 int launch_collapse(qsv_state *s, int q, int outcome, double scale);
 int marginals(qsv_state *s, const int *qubits, int m, double *out);
 

@@ -24,7 +24,7 @@ EXPORTED = [
     "qsv_cphase", "qsv_cr", "qsv_iswap", "qsv_sqiswap", "qsv_cswap", "qsv_apply_2q",
     "qsv_apply_block", "qsv_prob0", "qsv_norm2", "qsv_collapse", "qsv_measure",
     "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
-    "qsv_sample_shots", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
+    "qsv_sample_shots", "qsv_sample_shots_copy", "qsv_gather", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
     "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan",
     "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
     "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
@@ -51,7 +51,7 @@ class RunStats(C.Structure):
                 ("tie_resolutions", C.c_int64), ("device_ms", C.c_double),
                 ("model_ms_pred", C.c_double), ("class_launches", C.c_int64 * 8),
                 ("class_ms", C.c_double * 8), ("class_bytes", C.c_double * 8),
-                ("class_flops", C.c_double * 8), ("plan_ms", C.c_double)]
+                ("class_flops", C.c_double * 8), ("plan_ms", C.c_double), ("fuse_k", C.c_int64)]
 
     CLASS_NAMES = ["dense1", "diag", "perm", "phase", "block64", "fused", "readout", "pass"]
 
@@ -115,6 +115,10 @@ def lib():
                                  vp, C.POINTER(RunStats)], I),
             "qsv_sample_shots": ([vp, C.POINTER(I), I, C.c_int64, C.POINTER(C.c_int32), C.c_int64,
                                   np.ctypeslib.ndpointer(dtype=np.uint64), C.POINTER(C.c_int64)], I),
+            "qsv_gather": ([vp, np.ctypeslib.ndpointer(dtype=np.uint64), C.c_int64,
+                            np.ctypeslib.ndpointer(dtype=np.float64)], I),
+            "qsv_sample_shots_copy": ([vp, C.POINTER(I), I, C.c_int64, C.POINTER(C.c_int32), C.c_int64,
+                                       np.ctypeslib.ndpointer(dtype=np.uint64), C.POINTER(C.c_int64)], I),
             "qsv_pm_next": ([C.c_int32], C.c_int32),
             "qsv_pm_jump": ([C.c_int32, U64], C.c_int32),
             "qsv_pm_seed_from_tm": ([I] * 6, C.c_int32),
@@ -303,16 +307,25 @@ class StateVector:
         _check(lib().qsv_pmeasure(self.h, _ints(qubits), len(qubits), idx, prob, C.byref(cnt)))
         return list(zip(idx[:cnt.value].tolist(), prob[:cnt.value].tolist()))
 
-    def sample_shots(self, qubits, gate_draws: int, rng_state: int, shots: int):
+    def gather(self, idx) -> np.ndarray:
+        """Amplitudes at the given basis indices (complex128)."""
+        idx = np.ascontiguousarray(idx, np.uint64)
+        out = np.zeros(2 * max(1, idx.size), np.float64)
+        _check(lib().qsv_gather(self.h, idx if idx.size else np.zeros(1, np.uint64), idx.size, out))
+        return out[: 2 * idx.size].view(np.complex128)
+
+    def sample_shots(self, qubits, gate_draws: int, rng_state: int, shots: int, method: str = "tree"):
         """Reference shot loop from the current state (which is left unchanged):
         each shot skips `gate_draws` draws, then measures `qubits` in order.
-        Returns (words, rng_state, tie_resolutions)."""
-        out = np.zeros(shots, np.uint64)
+        method "tree": prefix-sum sampler (no state copy per shot); "copy": copy
+        and measure per shot, as the reference's loop.  Both give the same
+        outcomes.  Returns (words, rng_state, ties)."""
+        out = np.zeros(max(shots, 1), np.uint64)
         rs = C.c_int32(rng_state)
         ties = C.c_int64(0)
-        _check(lib().qsv_sample_shots(self.h, _ints(qubits), len(qubits), gate_draws, C.byref(rs),
-                                      shots, out, C.byref(ties)))
-        return out, rs.value, ties.value
+        fn = {"tree": lib().qsv_sample_shots, "copy": lib().qsv_sample_shots_copy}[method]
+        _check(fn(self.h, _ints(qubits), len(qubits), gate_draws, C.byref(rs), shots, out, C.byref(ties)))
+        return out[:shots], rs.value, ties.value
 
     def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=True, scheduler=0,
             rng_state=None, outcomes=None, profile=False, tile=0, group=0, pass_tb=0) -> RunStats:

@@ -60,8 +60,9 @@ def test_golden_state_c128(case):
     assert np.abs(got - ref).max() <= TOL
 
 
+@pytest.mark.parametrize("method", ["tree", "copy"])
 @pytest.mark.parametrize("case", ["cfg0_random_n16_d20", "hadamard_measure_n4"])
-def test_shots_bit_exact(case):
+def test_shots_bit_exact(case, method):
     meta = manifest()[case]
     circ = C.Circuit.load(os.path.join(GOLDEN, case + ".qsvc"))
     ref = np.fromfile(os.path.join(GOLDEN, case + ".shots"), np.uint64)
@@ -71,7 +72,7 @@ def test_shots_bit_exact(case):
     say("shots", case, "n", circ.n, "shots", meta["shots"])
     with E.StateVector(circ.n) as sv:
         sv.run(pre)
-        words, rng, ties = sv.sample_shots(mq, pre.ngates, meta["seed"], meta["shots"])
+        words, rng, ties = sv.sample_shots(mq, pre.ngates, meta["seed"], meta["shots"], method=method)
     say("tie resolutions", ties)
     assert np.array_equal(words, ref)
 
