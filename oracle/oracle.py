@@ -208,6 +208,28 @@ def mlp_grad(params, x, t):
     return g, loss
 
 
+def adam_update(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step with bias correction (Kingma & Ba, ICLR 2015, Algorithm 1),
+    FP64: returns the new (p, m, v).  `step` counts from 1."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mhat = m / (1 - b1 ** step)
+    vhat = v / (1 - b2 ** step)
+    return p - lr * mhat / (np.sqrt(vhat) + eps), m, v
+
+
+def mlp_train(params, x, t, epochs, lr):
+    """Full-batch Adam on 0.5 * mean((y - t)^2) from zero moments: the training
+    loop of the cost model, restated in FP64.  Returns the parameters."""
+    p = np.array(params, np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in range(1, epochs + 1):
+        g, _ = mlp_grad(p, x, t)
+        p, m, v = adam_update(p, g, m, v, step, lr)
+    return p
+
+
 def pm_next(s: int) -> int:
     return lib().orc_pm_next(s)
 

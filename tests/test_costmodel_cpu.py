@@ -64,3 +64,25 @@ def test_block_features():
     f5 = E.block_features(28, 5, 0, 8, [0, 1, 2, 3, 4])
     assert f5[3] == 32 / 128 and f5[4] == 0 and abs(f5[5] - 3 / 5) < 1e-7
     assert f5[6] == 1.0 and f5[8] == 1.0 and f5[11] == 0
+
+
+def test_adam_closed_forms():
+    # with a constant gradient the bias-corrected moments are exactly g and g^2,
+    # so every step moves each parameter by -lr * g / (|g| + eps); a dropped
+    # bias correction or a swapped moment fails this
+    g = np.array([3.0, -2e-3, 0.0, 7e-9, -1.5])
+    p0 = np.array([0.1, -0.2, 0.3, 0.4, 0.5])
+    p, m, v = p0.copy(), np.zeros(5), np.zeros(5)
+    lr = 3e-3
+    for step in range(1, 6):
+        p, m, v = O.adam_update(p, g, m, v, step, lr)
+        expect = p0 - step * lr * g / (np.abs(g) + 1e-8)
+        assert np.allclose(p, expect, rtol=0, atol=1e-15), step
+    # and the first step of a real training run is p - lr * g / (|g| + eps)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1, (40, 12))
+    t = rng.normal(size=40)
+    params = rng.normal(scale=0.2, size=NP)
+    g, _ = O.mlp_grad(params, x, t)
+    p1 = O.mlp_train(params, x, t, 1, lr)
+    assert np.allclose(p1, params - lr * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-14)

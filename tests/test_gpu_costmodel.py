@@ -47,6 +47,28 @@ def test_gradients_match_oracle():
     assert abs(loss - lref) < 1e-5 * max(1.0, lref)
 
 
+def test_device_adam_steps_match_oracle():
+    # three full-batch Adam steps on the device against the FP64 restatement
+    m = E.CostModel(21)
+    x, t = batch(5, 300)
+    p0 = m.get_params().astype(np.float64)
+    m.train(x, t, 3, 3e-3)
+    got = m.get_params().astype(np.float64)
+    lr = 3e-3
+    ref = O.mlp_train(p0, x.astype(np.float64), t.astype(np.float64), 3, lr)
+    # where the gradient is tiny, g / (|g| + eps) is ill-conditioned between the
+    # device's FP32 gradient and the FP64 one: check those only against Adam's
+    # step bound (lr (1 - b1) / sqrt(1 - b2) ~ 3.2 lr per step); the rest to 1%
+    # of one step
+    g0, _ = O.mlp_grad(p0, x.astype(np.float64), t.astype(np.float64))
+    well = np.abs(g0) > 1e-3 * np.abs(g0).max()
+    err = np.abs(got - ref)
+    say("max |params - oracle| after 3 Adam steps", err[well].max(), "on", int(well.sum()), "of", err.size)
+    assert well.sum() > err.size // 4
+    assert err[well].max() < 3e-5
+    assert np.abs(got - p0).max() <= 3 * 3.17 * lr
+
+
 def test_training_on_device_reduces_loss():
     m = E.CostModel(13)
     x, t = batch(3, 600)
