@@ -24,7 +24,14 @@ Other workloads (one JSON line each; not the default line):
                       the 28-qubit restriction of the same seeded generator;
   --workload batch    4096 hardware-efficient-ansatz instances (10..22 qubits,
                       1..8 layers) in one persistent launch under hipGraph
-                      replay; instances/s with per-instance p50 / p99 latency.
+                      replay; instances/s with per-instance p50 / p99 latency;
+  --workload qrunes   the random-circuit generator at n=26, depth 20 (no Haar
+                      pairs; CZ written as CR(pi)) written as a QRunes program
+                      with MEASURE on every qubit, run through the QPandaSDK
+                      facade (load, setComputeUnit(GPU), run(10^5)): shots/s.
+                      The reference re-runs the program per shot; the facade
+                      runs it once and samples the shots, and the bench checks
+                      on a short run that both give the same result strings.
 
 Roofline: the dominant kernel class of the headline path, timed per launch with
 HIP events on the state's stream inside the timed steps; achieved = algorithmic
@@ -62,10 +69,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", choices=["random30", "large33", "batch"], default="random30")
+    p.add_argument("--workload", choices=["random30", "large33", "batch", "qrunes"], default="random30")
     p.add_argument("--n", type=int, default=None, help="qubits (random30: 30, large33: 33)")
     p.add_argument("--depth", type=int, default=None, help="layers (random30: 200, large33: 60)")
-    p.add_argument("--shots", type=int, default=100000, help="large33: measurement shots")
+    p.add_argument("--shots", type=int, default=100000, help="large33, qrunes: measurement shots")
     p.add_argument("--count", type=int, default=4096, help="batch: instances")
     p.add_argument("--haar", type=float, default=0.1)
     p.add_argument("--seed", type=int, default=2025)
@@ -84,9 +91,9 @@ def parse():
                    help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
     a = p.parse_args()
     if a.n is None:
-        a.n = 33 if a.workload == "large33" else 30
+        a.n = {"large33": 33, "qrunes": 26}.get(a.workload, 30)
     if a.depth is None:
-        a.depth = 60 if a.workload == "large33" else 200
+        a.depth = {"large33": 60, "qrunes": 20}.get(a.workload, 200)
     return a
 
 
@@ -235,6 +242,8 @@ def main():
         return main_large(a)
     if a.workload == "batch":
         return main_batch(a)
+    if a.workload == "qrunes":
+        return main_qrunes(a)
     from qpanda_amd import circuits as C
     from qpanda_amd import engine as E
 
@@ -462,6 +471,89 @@ def main_batch(a):
         "vs_baseline": None, "dtype": "c64 (fp32 state, fp64 reductions)", "data": "synthetic seeded instances",
         "config": {"workload": f"{a.count} hardware-efficient-ansatz instances seed={a.seed}",
                    "instances": a.count}, "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+def main_qrunes(a):
+    """A QRunes program through the QPandaSDK facade: shots/s of run(shots)."""
+    from qpanda_amd import circuits as C
+    from qpanda_amd import engine as E
+    from qpanda_amd import sdk
+    circ = C.random_layers(a.n, a.depth, a.seed, haar_frac=0.0)
+    G = circ.ngates
+    steps = a.steps if a.steps is not None else 3
+    warmup = a.warmup if a.warmup is not None else 1
+    log(f"qrunes workload n={a.n} depth={a.depth} gates={G} shots={a.shots}")
+    result = {}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "program.txt")
+        with open(path, "w") as f:
+            f.write(C.to_qrunes(circ))
+
+        def session(seed, resample_off=False):
+            s = sdk.SDK()
+            if s.load_file(path) != sdk.Q_ERROR_NONE or s.set_compute_unit(sdk.GPU) != sdk.Q_ERROR_NONE:
+                raise SystemExit("QPandaSDK: loading the program failed")
+            s.set_seed(seed)
+            s.set_resample_disabled(resample_off)
+            return s
+
+        s = session(a.seed)
+        for _ in range(warmup):
+            if s.run(a.shots) != sdk.Q_ERROR_NONE:
+                raise SystemExit("QPandaSDK: run failed")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if s.run(a.shots) != sdk.Q_ERROR_NONE:
+                raise SystemExit("QPandaSDK: run failed")
+        wall = time.perf_counter() - t0
+        if not s.last_run_sampled():
+            raise SystemExit("QPandaSDK: the program did not take the sampled path")
+        result["distinct_outcomes"] = s.result().count("\n")
+        s.close()
+        # the reference's loop: the whole program again for every shot
+        rerun_shots = 2
+        s1, s2 = session(a.seed + 1), session(a.seed + 1, resample_off=True)
+        s1.run(rerun_shots)
+        t1 = time.perf_counter()
+        s2.run(rerun_shots)
+        rerun_s = (time.perf_counter() - t1) / rerun_shots
+        result["rerun_equals_sampled"] = bool(s1.result() == s2.result() and
+                                              np.array_equal(s1.amplitudes(), s2.amplitudes()))
+        result["rerun_shots_per_s"] = 1.0 / rerun_s
+        s1.close()
+        s2.close()
+    # the facade dispatches one FP64 kernel per gate; the same kernels, timed per
+    # launch with HIP events, in a run of the same gates through qsv_run_circuit
+    with E.StateVector(a.n, E.C128) as sv:
+        st = sv.run(circ, profile=True)
+        sv.sync()
+    roof = roofline_of([st], a, "plain", result)
+    roof["note"] = ("per-gate FP64 kernels of the program, timed in a run of the same gates through the "
+                    "engine's FP64 path (the facade launches the same kernels, one call per gate)")
+    result["gates"] = G
+    result["speedup_vs_rerun"] = (steps * a.shots / wall) / result["rerun_shots_per_s"]
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline")
+        cg = cpu_baseline(circ, a.cpu_seconds, 1)
+        cpu = dict(cg)
+        cpu["value"] = cg["value"] / G
+        cpu["unit"] = "shots/s"
+        cpu["sample"] = (cg["sample"] + f"; shots/s = gate-apps/s / {G} gates, since QPandaSDK::run "
+                         "executes the whole program for every shot (its measurements not counted)")
+    line = {
+        "metric": f"shots/s (QRunes program through QPandaSDK: n={a.n}, depth {a.depth}, MEASURE all, "
+                  f"{a.shots} shots per run)",
+        "value": steps * a.shots / wall, "unit": "shots/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (fp64)", "data": "synthetic seeded circuit as QRunes text",
+        "config": {"workload": f"QRunes program: random circuit n={a.n} depth={a.depth} seed={a.seed} "
+                               f"({G} gates, CZ as CR(pi)) + {a.n} MEASURE, run({a.shots})",
+                   "n_qubits": a.n, "depth": a.depth, "shots": a.shots},
+        "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(result)
     print(json.dumps(line), flush=True)

@@ -281,3 +281,107 @@ def ref_time(circ_path: str, stride: int, max_seconds: float, omp: bool = True,
         env["OMP_NUM_THREADS"] = str(threads)
     out = subprocess.check_output([exe, "time", circ_path, str(stride), str(max_seconds)], env=env)
     return json.loads(out.decode().strip().splitlines()[-1])
+
+
+# ---- QRunes programs: executor and run loop -------------------------------
+
+def _qrunes_node(st: "State", nd: dict, rng: int, regs: dict, pm: list):
+    """Apply one flattened QRunes instruction the way the reference executor
+    does (Instruction.cpp:40-827 dispatch, CPUQuantumGates draws): returns
+    (ok, rng).  Every gate call draws once and applies when the draw exceeds
+    error_rate = 0; the controlled iSWAP / sqiSWAP overloads are no-ops that
+    draw nothing (CPUQuantumGates.cpp:914-922, 1054-1062); a measurement draws
+    once; Reset measures and, on 1, applies NOT (one more draw)."""
+    k, qs, ctl = nd["kind"], nd["qubits"], list(nd["controls"])
+    dag = 1 if nd["dagger"] else 0
+    if any(q >= st.n for q in qs):
+        return False, rng
+
+    def draw():
+        nonlocal rng
+        rng = pm_next(rng)
+        return rng / 2147483647.0
+
+    if k == "H":
+        draw()
+        if ctl:
+            st.controlled("H", ctl + [qs[0]])
+        else:
+            st.h(qs[0])
+    elif k in ("RX", "RY", "RZ"):
+        draw()
+        if ctl:
+            st.controlled(k, ctl + [qs[0]], theta=nd["angle"], dagger=dag)
+        else:
+            getattr(st, k.lower())(qs[0], nd["angle"], dag)
+    elif k == "NOT":
+        draw()
+        if ctl:
+            st.controlled("X", ctl + [qs[0]])
+        else:
+            st.x(qs[0])
+    elif k == "CNOT":                 # controlled form: NOT on the target (CPUQuantumGates.cpp:750-757)
+        draw()
+        if ctl:
+            st.controlled("X", ctl + [qs[0], qs[1]])
+        else:
+            st.cnot(qs[0], qs[1])
+    elif k == "TOFFOLI":              # Instruction.cpp:378-386
+        draw()
+        if ctl:
+            st.controlled("X", ctl + qs)
+        else:
+            st.toffoli(qs[0], qs[1], qs[2])
+    elif k == "CR":                   # Instruction.cpp:517-526: controls ignored
+        draw()
+        st.cr(qs[0], qs[1], nd["angle"], dag)
+    elif k == "ISWAP":                # Instruction.cpp:558-577: branches swapped
+        if ctl:
+            draw()
+            st.iswap(qs[0], qs[1], dag)
+    elif k == "SQISWAP":
+        if not ctl:
+            draw()
+            st.sqiswap(qs[0], qs[1], dag)
+    elif k == "RESET":                # CPUQuantumGates.cpp:126-136
+        if st.measure(qs[0], draw()):
+            draw()
+            st.x(qs[0])
+    elif k == "MEASURE":
+        regs[nd["creg"]] = st.measure(qs[0], draw())
+    elif k == "PMEASURE":
+        m = st.marginals(qs)
+        pm[:] = [len(qs), m]
+    else:
+        return False, rng
+    return True, rng
+
+
+def qrunes_run(n: int, nodes, repeat: int, rng: int):
+    """QPandaSDK::run (QPandaAPI.cpp:123-205) over the oracle state: returns
+    (result lines as getResult prints them, final state, generator state).
+    PMEASURE results come back as {index: probability} for the caller to
+    compare (the reference keeps the top 10 after an unstable sort)."""
+    counts = {}
+    regs = {}
+    pm = []
+    st = State(n)
+    for run in range(repeat):
+        st = State(n)
+        for nd in nodes:
+            ok, rng = _qrunes_node(st, nd, rng, regs, pm)
+            if not ok:
+                raise ValueError("oracle: instruction failed")
+        if pm:
+            break
+        key = "".join(str(regs[c]) for c in sorted(regs))[::-1]
+        counts[key] = counts.get(key, 0) + 1
+    if pm:
+        return {i: float(p) for i, p in enumerate(pm[1])}, st, rng
+    lines = "".join(f"{k} {_cpp_g(v / repeat)}\n" for k, v in sorted(counts.items()))
+    return lines, st, rng
+
+
+def _cpp_g(x: float) -> str:
+    """std::ostream's default formatting of a double (printf %g, 6 digits)."""
+    return "%g" % x

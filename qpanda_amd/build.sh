@@ -30,6 +30,12 @@ done
 "$HIPCC" -shared -fPIC --offload-arch=gfx950 -o libqsv.so.tmp "${OBJS[@]}"
 mv libqsv.so.tmp libqsv.so
 echo "built $HERE/libqsv.so"
+# QRunes front end + SDK facade over the QuantumGates adapter (plain C++ over the C ABI)
+g++ -O2 -std=c++14 -Wall -fPIC -shared -I"$HERE/cpp" -I"$HERE/../include" -o libqpanda_sdk.so.tmp \
+    cpp/QRunes.cpp cpp/QPandaSDK.cpp cpp/HipQuantumGates.cpp cpp/QuantumGates.cpp \
+    -L"$HERE" -lqsv -Wl,-rpath,'$ORIGIN'
+mv libqpanda_sdk.so.tmp libqpanda_sdk.so
+echo "built $HERE/libqpanda_sdk.so"
 # QuantumGates adapter (plain C++ over the C ABI) and its known-answer test driver
 g++ -O2 -std=c++14 -Wall -I"$HERE/cpp" -I"$HERE/../include" -o _build/test_hip_quantum_gates \
     "$HERE/../tests/cpp/test_hip_quantum_gates.cpp" cpp/HipQuantumGates.cpp cpp/QuantumGates.cpp \

@@ -213,6 +213,43 @@ def hea(n: int, layers: int, seed: int) -> Circuit:
     return b.build(f"hea_n{n}_l{layers}_s{seed}")
 
 
+def to_qrunes(circ: Circuit, measure: bool = True) -> str:
+    """QRunes text of a circuit over the QRunes gate set (QRunes/README.md;
+    tokens of QRunesParser/lex.yy.cpp:914-1079): H, NOT, RX/RY/RZ, CNOT, CR
+    (CZ is written as CR with angle pi), ISWAP, SQISWAP, TOFFOLI, with
+    MEASURE q,$q on every qubit when `measure`.  Angles are written as fixed
+    point decimals with 25 fractional digits, which read back as the same
+    doubles; daggered records use a DAGGER block."""
+    out = [f"QINIT {circ.n}", f"CREG {circ.n}"]
+    for g in circ.gates:
+        k, q0, q1, q2, th = int(g["kind"]), int(g["q0"]), int(g["q1"]), int(g["q2"]), float(g["theta"])
+        ang = f'"{th:.25f}"'
+        if k == H:
+            line = f"H {q0}"
+        elif k == X:
+            line = f"NOT {q0}"
+        elif k in (RX, RY, RZ):
+            line = f"{KIND_NAMES[k]} {q0},{ang}"
+        elif k == CNOT:
+            line = f"CNOT {q0},{q1}"
+        elif k == CZ:
+            line = f'CR {q0},{q1},"pi"'
+        elif k == CR:
+            line = f"CR {q0},{q1},{ang}"
+        elif k == ISWAP:
+            line = f'ISWAP {q0},{q1},"0"'
+        elif k == SQISWAP:
+            line = f"SQISWAP {q0},{q1}"
+        elif k == TOFFOLI:
+            line = f"TOFFOLI {q0},{q1},{q2}"
+        else:
+            raise ValueError(f"{KIND_NAMES.get(k, k)} has no QRunes statement")
+        out += ["DAGGER", line, "ENDDAGGER"] if int(g["dagger"]) else [line]
+    if measure:
+        out += [f"MEASURE {q},${q}" for q in range(circ.n)]
+    return "\n".join(out) + "\n"
+
+
 def random_gaussian_state(n: int, seed: int) -> np.ndarray:
     """Normalised state with i.i.d. Gaussian real and imaginary parts, from a
     seeded numpy PCG64 stream (vectorised; 2^26 amplitudes in about a second)."""

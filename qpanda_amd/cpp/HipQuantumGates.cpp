@@ -18,6 +18,7 @@ double HipQuantumGates::draw()
 {
     if (!mSeeded) return randGenerator();
     mRng = qsv_pm_next(mRng);
+    mDraws++;
     return (double)mRng / 2147483647;
 }
 

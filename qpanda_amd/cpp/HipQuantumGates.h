@@ -64,6 +64,11 @@ public:
     // Use a private Park-Miller stream starting at `seed` instead of the
     // process-wide time-seeded one (for reproducible runs).
     void seedRandom(int32_t seed);
+    // State of that private stream and the number of draws taken from it
+    // (both valid after seedRandom).
+    int32_t rngState() const { return mRng; }
+    void setRngState(int32_t s) { mRng = s; }
+    int64_t draws() const { return mDraws; }
     qsv_state *handle() { return mState; }
 
 protected:
@@ -79,6 +84,7 @@ private:
     int mQubits = 0;
     bool mSeeded = false;
     int32_t mRng = 1;
+    int64_t mDraws = 0;
     std::string sCalculationUnitType = "HIP";
 };
 

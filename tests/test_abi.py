@@ -41,3 +41,21 @@ def test_rng_entry_points_are_pure_host_code():
     for _ in range(10000):
         s = E.pm_next(s)
     assert s == 1043618065 and E.pm_jump(1, 10000) == 1043618065
+
+
+def sdk_declared():
+    src = open(os.path.join(ROOT, "include", "qsv_sdk.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", src)) - {"extern"})
+
+
+def test_sdk_library_exports_every_declared_symbol():
+    from qpanda_amd import sdk
+    names = sdk_declared()
+    assert {"loadFileAPI", "setComputeUnitAPI", "runAPI", "getResultAPI", "getQuantumStateAPI",
+            "qrunes_parse_file", "qpanda_sdk_run"} <= set(names)
+    lib = sdk.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the binding declares a signature for every entry point
+    assert [n for n in names if getattr(lib, n).argtypes is None] == []
