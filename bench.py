@@ -8,10 +8,13 @@ two-qubit unitaries), 9,000 logical gates, synthetic, generated from a fixed see
 
 Paths:
   plain  per-gate complex128 sweeps (one launch per gate), the FP64 reference path;
-  fused  complex64 state, gates fused into <= k-qubit blocks run as FP32 MFMA
-         complex GEMMs with the hi+lo split matrix product, scheduled by the
-         trained cost model (block size, grouping of blocks into shared HBM
-         sweeps, tile shape); --scheduler greedy gives one sweep per block.
+  fused  complex64 state, gates fused into <= k-qubit blocks run as complex
+         GEMMs on the matrix cores -- by default the BF16x6 product (matrix and
+         state split into three bf16 pieces, six piece products accumulated in
+         FP32; --product split: FP32 MFMA plus a BF16 correction of the
+         matrix's lo part; --product fp32) -- scheduled by the trained cost
+         model (block size, grouping of blocks into shared HBM sweeps, tile
+         shape); --scheduler greedy gives one sweep per block.
 A step is one full circuit execution from |0...0> on the headline path (fused).
 The plain path and the greedy k=4 fused path each run once for comparison;
 fidelity |<plain|fused>|^2 is computed on the device in FP64.
@@ -31,15 +34,21 @@ Other workloads (one JSON line each; not the default line):
                       facade (load, setComputeUnit(GPU), run(10^5)): shots/s.
                       The reference re-runs the program per shot; the facade
                       runs it once and samples the shots, and the bench checks
-                      on a short run that both give the same result strings.
+                      on a short run that both give the same result strings;
+  --workload stateio  whole-state host copies and binary state files of a
+                      28-qubit complex128 state (4 GiB) and its complex64 form:
+                      GB/s of get / set / save / load, checked bit-exact.
 
 Roofline: the dominant kernel class of the headline path, timed per launch with
 HIP events on the state's stream inside the timed steps; achieved = algorithmic
 bytes (every amplitude a gate may change, read and written once) / time.  A
-fused k=4 block performs 8 flop per byte moved, below the FP32 MFMA ridge (~20
-flop/B), so a single-block sweep is HBM-bound; a multi-block sweep ("pass")
-performs 8 flop/B per block and is MFMA-bound once its intensity crosses the
-ridge.  The FP32 MFMA rate is always reported ("mfma").  "copy_gbs" is a
+fused k=4 block performs 8 complex-GEMM flop per byte moved; a multi-block
+sweep ("pass") 8 flop/B per block.  The matrix-core peak in those flops is
+the FP32 MFMA peak (157.3 TF, ridge ~20 flop/B) for the FP32 products and the
+BF16 dense peak / 6 (417 TF, ridge ~52 flop/B) for BF16x6, which executes
+six bf16 piece products per flop; a sweep is MFMA-bound once its intensity
+crosses the ridge, else HBM-bound.  The matrix-core rate is always reported
+("mfma").  "copy_gbs" is a
 state-to-state copy of the same size timed in the same run: the practical
 streaming rate of the device, for reference.
 
@@ -62,6 +71,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -69,7 +79,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", choices=["random30", "large33", "batch", "qrunes"], default="random30")
+    p.add_argument("--workload", choices=["random30", "large33", "batch", "qrunes", "stateio"], default="random30")
     p.add_argument("--n", type=int, default=None, help="qubits (random30: 30, large33: 33)")
     p.add_argument("--depth", type=int, default=None, help="layers (random30: 200, large33: 60)")
     p.add_argument("--shots", type=int, default=100000, help="large33, qrunes: measurement shots")
@@ -83,6 +93,8 @@ def parse():
     p.add_argument("--group", type=int, default=8, help="blocks per sweep for --scheduler passes")
     p.add_argument("--costmodel", default=None, help="trained cost-model parameters (float32)")
     p.add_argument("--path", choices=["fused", "plain"], default="fused")
+    p.add_argument("--product", choices=["fp32", "split", "bf16x6"], default="bf16x6",
+                   help="matrix product of the fused blocks")
     p.add_argument("--no-plain", action="store_true", help="skip the FP64 and greedy comparison runs")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -90,8 +102,9 @@ def parse():
     p.add_argument("--probe", action="store_true",
                    help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
     a = p.parse_args()
+    a.split = {"fp32": 0, "split": 1, "bf16x6": 2}[a.product]
     if a.n is None:
-        a.n = {"large33": 33, "qrunes": 26}.get(a.workload, 30)
+        a.n = {"large33": 33, "qrunes": 26, "stateio": 28}.get(a.workload, 30)
     if a.depth is None:
         a.depth = {"large33": 60, "qrunes": 20}.get(a.workload, 200)
     return a
@@ -125,7 +138,7 @@ def measured_traffic(kernel_class, a, headline, k):
     calibration), when they cover this kernel at this size; else None."""
     key = None
     if kernel_class in ("fused", "pass"):
-        key = f"{kernel_class}_k{k}_split@{a.n}"
+        key = f"{kernel_class}_k{k}_{a.product}@{a.n}"
     elif kernel_class == "dense1" and headline == "plain":
         key = f"dense1_c128@{a.n}"
     try:
@@ -193,6 +206,12 @@ def load_costmodel(a):
     return E.CostModel.load(path)
 
 
+def fused_dtype(a):
+    return {"bf16x6": "c64 (fp32 state; bf16x6 matrix-core product, fp32 accumulation; fp64 reductions)",
+            "split": "c64 (fp32 mfma + bf16 correction; fp64 reductions)",
+            "fp32": "c64 (fp32 mfma; fp64 reductions)"}[a.product]
+
+
 def roofline_of(stats_list, a, headline, result):
     """Dominant kernel class over the given runs: roofline object, MFMA rate."""
     agg = {}
@@ -206,12 +225,18 @@ def roofline_of(stats_list, a, headline, result):
     achieved = d["bytes"] / (d["ms"] * 1e6)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     if dom in ("fused", "pass"):
+        # flops: the complex GEMM's own (8 per amplitude and block qubit set);
+        # BF16x6 executes them six times over on the BF16 matrix cores, so its
+        # peak in these flops is the BF16 dense peak / 6
         tf = d["flops"] / (d["ms"] * 1e9)
         intensity = d["flops"] / d["bytes"]
-        result["mfma"] = {"fp32_tflops": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
-                          "flop_per_byte": intensity}
-        if intensity * HBM_PEAK_GBS / 1e3 >= FP32_PEAK_TFLOPS:      # right of the FP32 MFMA ridge
-            roof = {"bound": "mfma", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        peak = BF16_PEAK_TFLOPS / 6 if a.product == "bf16x6" else FP32_PEAK_TFLOPS
+        result["mfma"] = {"tflops": tf, "peak": peak, "frac": tf / peak, "flop_per_byte": intensity,
+                          "product": a.product}
+        if a.product == "bf16x6":
+            result["mfma"]["bf16_tflops_executed"] = 6 * tf
+        if intensity * HBM_PEAK_GBS / 1e3 >= peak:                  # right of the ridge
+            roof = {"bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     k_used = int(stats_list[-1].fuse_k) or a.fuse_k
     roof["traffic"] = measured_traffic(dom, a, headline, k_used)
@@ -244,6 +269,8 @@ def main():
         return main_batch(a)
     if a.workload == "qrunes":
         return main_qrunes(a)
+    if a.workload == "stateio":
+        return main_stateio(a)
     from qpanda_amd import circuits as C
     from qpanda_amd import engine as E
 
@@ -289,25 +316,25 @@ def main():
         k = a.fuse_k
         if sv_plain is not None and sched != 0:
             log("greedy k=4 fused path: one run (comparison)")
-            greedy = sv.run(circ, fuse_k=4, split=True, scheduler=0, profile=True)
+            greedy = sv.run(circ, fuse_k=4, split=a.split, scheduler=0, profile=True)
             result["greedy_k4"] = {"gate_apps_per_s": G / (greedy.device_ms / 1e3), "device_ms": greedy.device_ms,
                                    "sweeps": int(greedy.launches), "fidelity_vs_fp64": sv_plain.fidelity(sv)}
         for _ in range(warmup):
             sv.reset()
-            sv.run(circ, fuse_k=k, split=True, scheduler=sched, group=a.group)
+            sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group)
         sv.sync()
         t0 = time.perf_counter()
         stats = []
         for _ in range(steps):
             sv.reset()
-            stats.append(sv.run(circ, fuse_k=k, split=True, scheduler=sched, group=a.group, profile=True))
+            stats.append(sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group, profile=True))
         sv.sync()
         wall = time.perf_counter() - t0
         head_stats = stats
         if sv_plain is not None:
             result["fidelity_vs_fp64"] = sv_plain.fidelity(sv)
         cl = stats[-1].classes()
-        result["fused"] = {"k": int(stats[-1].fuse_k), "split": True, "scheduler": a.scheduler,
+        result["fused"] = {"k": int(stats[-1].fuse_k), "product": a.product, "scheduler": a.scheduler,
                            "sweeps": int(sum(cl.get(c, {}).get("launches", 0) for c in ("fused", "pass", "block64"))),
                            "plan_ms": stats[-1].plan_ms,
                            "blocks": int(stats[-1].blocks),
@@ -345,7 +372,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (fp32 mfma, fp64 reductions)" if headline == "fused" else "c128 (fp64)",
+        "dtype": fused_dtype(a) if headline == "fused" else "c128 (fp64)",
         "data": "synthetic seeded circuit",
         "config": {"workload": f"random circuit n={a.n} depth={a.depth} haar_frac={a.haar} "
                                f"seed={a.seed} ({G} gates)",
@@ -372,13 +399,13 @@ def main_large(a):
         sv.attach_costmodel(cm)
         for _ in range(warmup):
             sv.reset()
-            sv.run(circ, fuse_k=5, split=True, scheduler=1)
+            sv.run(circ, fuse_k=5, split=a.split, scheduler=1)
         sv.sync()
         stats = []
         t0 = time.perf_counter()
         for i in range(steps):
             sv.reset()
-            stats.append(sv.run(circ, fuse_k=5, split=True, scheduler=1, profile=True))
+            stats.append(sv.run(circ, fuse_k=5, split=a.split, scheduler=1, profile=True))
             log(f"step {i}: device {stats[-1].device_ms:.0f} ms")
         sv.sync()
         wall = time.perf_counter() - t0
@@ -386,7 +413,7 @@ def main_large(a):
         norm = sv.norm2()
         result["norm2_fp64"] = norm
         cl = st.classes()
-        result["fused"] = {"k": int(st.fuse_k), "split": True, "scheduler": "learned", "blocks": int(st.blocks),
+        result["fused"] = {"k": int(st.fuse_k), "product": a.product, "scheduler": "learned", "blocks": int(st.blocks),
                            "sweeps": int(sum(cl.get(c, {}).get("launches", 0) for c in ("fused", "pass", "block64"))),
                            "plan_ms": st.plan_ms, "device_ms": st.device_ms, "predicted_ms": st.model_ms_pred}
         # shots: all qubits in order from the final state, one draw per gate skipped per shot
@@ -406,7 +433,7 @@ def main_large(a):
     log("28-qubit restriction: learned c64 vs plain c128")
     with E.StateVector(nr, E.C64) as f, E.StateVector(nr, E.C128) as pl:
         f.attach_costmodel(cm)
-        f.run(c28, fuse_k=5, split=True, scheduler=1)
+        f.run(c28, fuse_k=5, split=a.split, scheduler=1)
         pl.run(c28)
         result["fidelity_vs_fp64_28q"] = pl.fidelity(f)
     cpu = None
@@ -417,7 +444,7 @@ def main_large(a):
         "metric": f"gate-apps/s at {a.n}q (random circuit, depth {a.depth}, complex64)",
         "value": G / (wall / steps), "unit": "gate-apps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 (fp32 mfma, fp64 reductions)", "data": "synthetic seeded circuit",
+        "dtype": fused_dtype(a), "data": "synthetic seeded circuit",
         "config": {"workload": f"random circuit n={a.n} depth={a.depth} haar_frac={a.haar} seed={a.seed} "
                                f"({G} gates), {a.shots} shots", "n_qubits": a.n, "depth": a.depth, "gates": G},
         "roofline": roof, "cpu_baseline": cpu,
@@ -556,6 +583,59 @@ def main_qrunes(a):
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+def main_stateio(a):
+    """Host copies and state files: GB/s of the state's FP64 bytes moved."""
+    from qpanda_amd import engine as E
+    steps = a.steps if a.steps is not None else 2
+    n = a.n
+    rng = np.random.default_rng(a.seed)
+    v = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)) / np.sqrt(2 << n)
+    gb = (16 << n) / 1e9
+    rates = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        path = os.path.join(td, "state.bin")
+        for prec, name in ((E.C128, "c128"), (E.C64, "c64")):
+            want = v if prec == E.C128 else v.astype(np.complex64).astype(np.complex128)
+            with E.StateVector(n, prec) as sv:
+                t = {}
+                for op in ("set", "get", "save", "load"):
+                    best = None
+                    for _ in range(steps):
+                        t0 = time.perf_counter()
+                        if op == "set":
+                            sv.set(v)
+                        elif op == "get":
+                            got = sv.get()
+                        elif op == "save":
+                            if os.path.exists(path):
+                                os.remove(path)        # every save creates the file
+                            sv.save(path)
+                        else:
+                            sv.load(path)
+                        dt = time.perf_counter() - t0
+                        best = dt if best is None else min(best, dt)
+                    t[op] = best
+                    if op == "get" and not np.array_equal(got, want):
+                        raise SystemExit(f"{name} get differs")
+                if not np.array_equal(sv.get(), want):
+                    raise SystemExit(f"{name} load differs")
+                rates[name] = {op: gb / t[op] for op in t}
+                log(f"{name}: " + " ".join(f"{op} {r:.1f} GB/s" for op, r in rates[name].items()))
+    value = 2 * gb / (gb / rates["c128"]["save"] + gb / rates["c128"]["load"])
+    line = {
+        "metric": f"GB/s of a binary state file written and read back (n={n}, c128, {gb:.1f} GB)",
+        "value": value, "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * (gb / rates["c128"]["save"] + gb / rates["c128"]["load"]),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 records",
+        "data": "synthetic random state", "config": {"workload": f"state I/O n={n}", "n_qubits": n},
+        "roofline": None, "cpu_baseline": None,
+        "rates_gbs": rates,
+        "note": "best of the timed repetitions; files in the host's temporary directory (page cache "
+                "included); rates count the state's FP64 bytes (16 per amplitude) for both precisions",
+    }
     print(json.dumps(line), flush=True)
 
 

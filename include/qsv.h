@@ -118,6 +118,22 @@ int qsv_precision(const qsv_state *s);
 int qsv_set_state(qsv_state *s, const double *amps);
 int qsv_get_state(qsv_state *s, double *amps);
 
+/* State files, streamed through pinned staging buffers (no whole-state host
+ * copy).  QSV_FILE_BINARY: 2^n records {f64 re, f64 im} in index order, the
+ * record layout of the reference CLI's .fullstate.binary
+ * (QCommand.cpp:437-487); QSV_FILE_TEXT: one line per amplitude,
+ * "state[i].real = X state[i].imag = Y", the getQState text
+ * (CPUQuantumGates.cpp:64-82) that the CLI saves as .fullstate
+ * (QCommand.cpp:402-428).  digits: significant digits of the text numbers
+ * (0 = 6, as the reference prints them; 17 reads back exactly); unused for
+ * binary.  A complex64 state is written as FP64 and read back rounded to FP32.
+ * Loading checks the size (binary) or every index (text); a failed load
+ * leaves the state partly overwritten.  The reference has no load path. */
+#define QSV_FILE_BINARY 0
+#define QSV_FILE_TEXT 1
+int qsv_save_state(qsv_state *s, const char *path, int format, int digits);
+int qsv_load_state(qsv_state *s, const char *path, int format);
+
 /* amps[2i], amps[2i+1] = amplitude idx[i] (FP64; indices < 2^n, checked).
  * No reference counterpart: reads single amplitudes of a state too large to
  * copy out, e.g. the probabilities of sampled outcomes. */
@@ -190,7 +206,9 @@ int qsv_apply_2q(qsv_state *s, int qa, int qb, const double *u);
 /* Dense k-qubit block (k = 1..5), u = 2^k x 2^k row-major complex doubles.
  * C128 states apply it in FP64; C64 states use the fused MFMA kernel
  * (FP32 complex GEMM with FP32 accumulation; split = 1 adds the lo part of
- * the matrix, U = U_hi + U_lo).  No reference counterpart. */
+ * the matrix, U = U_hi + U_lo; split = 2 runs the product as BF16x6 on the
+ * BF16 matrix cores: matrix and state split into three bf16 pieces, the six
+ * leading piece products accumulated in FP32).  No reference counterpart. */
 int qsv_apply_block(qsv_state *s, const int *qubits, int k, const double *u, int split);
 
 /* ---- readout --------------------------------------------------------------- */
@@ -261,7 +279,8 @@ int qsv_sample_shots_copy(qsv_state *s, const int *qubits, int m, int64_t gate_d
 typedef struct qsv_run_opts {
     int exact;          /* 1: H / sqiSWAP use exact 1/sqrt(2)                   */
     int fuse_k;         /* 0: per-gate path; 2..5: fused blocks of <= k qubits  */
-    int split;          /* fused blocks: 1 = hi+lo split matrix product         */
+    int split;          /* fused blocks: 0 = FP32 product, 1 = FP32 + BF16 lo   */
+                        /* correction, 2 = BF16x6 product                       */
     int scheduler;      /* 0: greedy fusion, 1: learned cost model              */
     int profile;        /* 1: time every launch with events (per-class stats)   */
     int tile;           /* fused blocks: tile column bits 6, 7 or 8 (0 = 7);   */

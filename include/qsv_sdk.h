@@ -57,6 +57,11 @@ int qpanda_sdk_run(qpanda_sdk *h, int repeat);
 int qpanda_sdk_last_run_sampled(qpanda_sdk *h);
 int qpanda_sdk_get_result(qpanda_sdk *h, char *buf, int *len);
 int qpanda_sdk_get_state(qpanda_sdk *h, char *buf, int *len);
+/* The reference CLI's state files of the last run's final state
+ * (QCommand.cpp:402-487): binary 0 writes path + ".fullstate" (the state text
+ * as getQuantumState prints it), 1 writes path + ".fullstate.binary" (2^n
+ * records {f64 re, f64 im}).  Returns a QError code. */
+int qpanda_sdk_save_state(qpanda_sdk *h, const char *path, int binary);
 /* FP64 amplitudes (re, im pairs) of the last run's final state: copies up to
  * count amplitudes into out and returns the state's length (0 before a run). */
 int64_t qpanda_sdk_get_amplitudes(qpanda_sdk *h, double *out, int64_t count);

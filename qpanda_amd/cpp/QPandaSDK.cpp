@@ -365,6 +365,16 @@ QStat QPandaSDK::amplitudes() const
     return mGates->state();
 }
 
+int QPandaSDK::saveState(const std::string &path, bool binary)
+{
+    if (path.empty()) return qParameterError;
+    if (!mbIsRun || !mGates || !mGates->handle()) return runProgramError;
+    const std::string file = path + (binary ? ".fullstate.binary" : ".fullstate");
+    if (qsv_save_state(mGates->handle(), file.c_str(), binary ? QSV_FILE_BINARY : QSV_FILE_TEXT, 0) != QSV_OK)
+        return runProgramError;
+    return qErrorNone;
+}
+
 }  // namespace QPanda
 
 using QPanda::QPandaSDK;
@@ -431,6 +441,10 @@ int qpanda_sdk_get_result(qpanda_sdk *h, char *buf, int *len)
 int qpanda_sdk_get_state(qpanda_sdk *h, char *buf, int *len)
 {
     return copy_out(reinterpret_cast<QPandaSDK *>(h)->getQuantumState(), buf, len);
+}
+int qpanda_sdk_save_state(qpanda_sdk *h, const char *path, int binary)
+{
+    return reinterpret_cast<QPandaSDK *>(h)->saveState(path ? std::string(path) : std::string(), binary != 0);
 }
 int64_t qpanda_sdk_get_amplitudes(qpanda_sdk *h, double *out, int64_t count)
 {

@@ -64,6 +64,11 @@ public:
     const qsvq::QRProgram &program() const { return mProg; }
     // FP64 amplitudes of the last run's final state (empty before a run).
     QStat amplitudes() const;
+    // The reference CLI's state files (QCommand.cpp:402-487): path + ".fullstate"
+    // (the getQuantumState text) or path + ".fullstate.binary" ({f64 re, f64 im}
+    // records), streamed from the device.  qErrorNone, or qParameterError for
+    // an empty path, runProgramError before a run or when writing fails.
+    int saveState(const std::string &path, bool binary);
 
 private:
     bool transmit();

@@ -145,37 +145,7 @@ int qsv_reset_zero(qsv_state *s)
 int qsv_num_qubits(const qsv_state *s) { return s ? s->n : -1; }
 int qsv_precision(const qsv_state *s) { return s ? s->prec : -1; }
 
-int qsv_set_state(qsv_state *s, const double *amps)
-{
-    CHECK_STATE(s);
-    if (!amps) return fail(QSV_ERR_ARG, "null amps");
-    if (s->prec == QSV_C128) {
-        QSV_HIP(hipMemcpyAsync(s->amps, amps, s->dim * 16, hipMemcpyHostToDevice, s->stream));
-    } else {
-        std::vector<float> tmp(2 * s->dim);
-        for (uint64_t i = 0; i < 2 * s->dim; i++) tmp[i] = (float)amps[i];
-        QSV_HIP(hipMemcpyAsync(s->amps, tmp.data(), s->dim * 8, hipMemcpyHostToDevice, s->stream));
-        QSV_HIP(hipStreamSynchronize(s->stream));
-    }
-    QSV_HIP(hipStreamSynchronize(s->stream));
-    return QSV_OK;
-}
-
-int qsv_get_state(qsv_state *s, double *amps)
-{
-    CHECK_STATE(s);
-    if (!amps) return fail(QSV_ERR_ARG, "null amps");
-    if (s->prec == QSV_C128) {
-        QSV_HIP(hipMemcpyAsync(amps, s->amps, s->dim * 16, hipMemcpyDeviceToHost, s->stream));
-        QSV_HIP(hipStreamSynchronize(s->stream));
-    } else {
-        std::vector<float> tmp(2 * s->dim);
-        QSV_HIP(hipMemcpyAsync(tmp.data(), s->amps, s->dim * 8, hipMemcpyDeviceToHost, s->stream));
-        QSV_HIP(hipStreamSynchronize(s->stream));
-        for (uint64_t i = 0; i < 2 * s->dim; i++) amps[i] = tmp[i];
-    }
-    return QSV_OK;
-}
+// qsv_set_state / qsv_get_state: qsv_io.cpp
 
 int qsv_gather(qsv_state *s, const uint64_t *idx, int64_t count, double *amps)
 {

@@ -215,7 +215,7 @@ void launch_features(int n, int split, const std::vector<const std::vector<int> 
     f[1] = single ? 1.0f : 0.0f;
     f[2] = G / 8.0f;
     f[3] = work / 128.0f;
-    f[4] = split ? 1.0f : 0.0f;
+    f[4] = (float)split;   // product mode 0, 1, 2
     f[5] = (tb - 10) / 5.0f;
     f[6] = (float)std::min(run, tb) / tb;
     f[7] = nu / 14.0f;

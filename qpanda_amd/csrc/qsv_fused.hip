@@ -30,6 +30,16 @@
 //          permuted (K-block sb, lane half hf, element e <-> real column
 //          2(8 sb + e) + hf) so that each lane's B values are exactly the FP32
 //          B values it already holds from the K-steps 8 sb .. 8 sb + 7;
+//          split = 2 (BF16x6) runs the whole product on the BF16 MFMA instead:
+//          M = a0 + a1 + a2 and each B value b = b0 + b1 + b2 are split into
+//          three bf16 pieces (b's split is exact: 3 x 8 significand bits
+//          cover FP32's 24), and the six pieces products a_i b_j with
+//          i + j <= 2 are accumulated in FP32, smallest first -- the
+//          BF16x6 scheme of Henry, Tang and Heinecke (ARITH 2019).  The
+//          dropped products are ~2^-24 of the result, the FP32 rounding
+//          level.  Per K of 16 real rows it costs 6 v_mfma_f32_32x32x16_bf16
+//          (6 x 32 cycles) against 8 v_mfma_f32_32x32x2f32 (8 x 64 cycles),
+//          2.7x less matrix-core time, plus the VALU work of splitting B;
 //   store: results go back to the planes, then 16-byte coalesced stores.
 // Per tile the kernel moves 2 * 8 * 2^(k+TC) bytes and performs
 // 2 * (2d)^2 * 2^TC flops in the FP32 product, so k = 4 is HBM-bound (8 flop/B,
@@ -56,8 +66,42 @@ struct FusedArgs {
     int col_rank[8];         // rank in L of the i-th free tile (column) bit
 };
 
+// acc[h] += M B over one K-block pair of bf16 pieces.  ap[p][h][sb]: piece p
+// of the A operand (permuted K order); bv: this lane's FP32 B values.
+template <int HALVES, int LOBLK>
+__device__ __forceinline__ void bf16x6_product(floatx16 (&acc)[HALVES], const bf16x8 (&ap)[3][HALVES][LOBLK],
+                                               const float *bv)
+{
+#pragma unroll
+    for (int sb = 0; sb < LOBLK; sb++) {
+        bf16x8 b0, b1, b2;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            // b = b0 + b1 + b2 exactly: each residual is exact in FP32
+            // (Sterbenz) and the last has at most 8 significant bits
+            const float x = bv[8 * sb + e];
+            const __bf16 t0 = (__bf16)x;
+            const float r1 = x - (float)t0;
+            const __bf16 t1 = (__bf16)r1;
+            b0[e] = t0;
+            b1[e] = t1;
+            b2[e] = (__bf16)(r1 - (float)t1);
+        }
+#pragma unroll
+        for (int h = 0; h < HALVES; h++) {
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2][h][sb], b0, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][h][sb], b1, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][h][sb], b2, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][h][sb], b0, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][h][sb], b1, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][h][sb], b0, acc[h], 0, 0, 0);
+        }
+    }
+}
+
 // KQ: block qubits (4, 5).  TC: column bits per tile (6, 7, 8 -> 2, 4, 8 waves).
-template <int KQ, int TC, bool SPLIT>
+// MODE: 0 FP32 product, 1 FP32 + BF16 lo correction, 2 BF16x6.
+template <int KQ, int TC, int MODE>
 __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, uint64_t ntiles, FusedArgs fa,
                                                          const float *__restrict__ mops)
 {
@@ -69,6 +113,8 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     constexpr int KSTEPS = D;                // K = 2D real rows, 2 per step
     constexpr int NLD = TSIZE / (2 * THREADS);   // 16-byte loads per thread
     constexpr int LOBLK = D / 8;             // bf16 K-blocks (16 real rows = 8 amplitudes)
+    constexpr bool SPLIT = MODE == 1;
+    constexpr bool X6 = MODE == 2;
     __shared__ float planes[2][TSIZE];     // [0] = re, [1] = im
     float *const pre = planes[0];
     float *const pim = planes[1];
@@ -89,12 +135,23 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     }
 
     // A operands (the matrix), operand-ready layout: mops[(h*KSTEPS + s)*64 + lane]
-    float ahi[HALVES][KSTEPS];
+    float ahi[X6 ? 1 : HALVES][X6 ? 1 : KSTEPS];
     bf16x8 alo[SPLIT ? HALVES : 1][SPLIT ? LOBLK : 1];
+    bf16x8 ap[3][HALVES][LOBLK];
+    if (X6) {
+        const bf16x8 *p = reinterpret_cast<const bf16x8 *>(mops);
 #pragma unroll
-    for (int h = 0; h < HALVES; h++)
+        for (int q = 0; q < 3; q++)
 #pragma unroll
-        for (int s = 0; s < KSTEPS; s++) ahi[h][s] = mops[(h * KSTEPS + s) * 64 + lane];
+            for (int h = 0; h < HALVES; h++)
+#pragma unroll
+                for (int sb = 0; sb < LOBLK; sb++) ap[q][h][sb] = p[((q * HALVES + h) * LOBLK + sb) * 64 + lane];
+    } else {
+#pragma unroll
+        for (int h = 0; h < HALVES; h++)
+#pragma unroll
+            for (int s = 0; s < KSTEPS; s++) ahi[X6 ? 0 : h][X6 ? 0 : s] = mops[(h * KSTEPS + s) * 64 + lane];
+    }
     if (SPLIT) {
         const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(mops + HALVES * KSTEPS * 64);
 #pragma unroll
@@ -127,10 +184,15 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     float bv[KSTEPS];
 #pragma unroll
     for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[fa.erow[s] | ecol];
+    if (X6) {
+        bf16x6_product<HALVES, LOBLK>(acc, ap, bv);
+    } else {
 #pragma unroll
-    for (int s = 0; s < KSTEPS; s++)
+        for (int s = 0; s < KSTEPS; s++)
 #pragma unroll
-        for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], bv[s], acc[h], 0, 0, 0);
+            for (int h = 0; h < HALVES; h++)
+                acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[X6 ? 0 : h][X6 ? 0 : s], bv[s], acc[h], 0, 0, 0);
+    }
     if (SPLIT) {
 #pragma unroll
         for (int sb = 0; sb < LOBLK; sb++) {
@@ -166,9 +228,13 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
 
 // Host-side: operand-ready matrices for a block.  u: d x d complex doubles
 // (row bit j = bit(qubits[j])).  Layout: HALVES * D * 64 floats of M_hi
-// (value M[32h + l%32][2s + l/32] at (h*D + s)*64 + l), then, with split,
+// (value M[32h + l%32][2s + l/32] at (h*D + s)*64 + l), then, with split = 1,
 // HALVES * (D/8) * 64 * 8 bf16 of M_lo in the permuted K order (value
 // M_lo[32h + l%32][2(8sb + e) + l/32] at ((h*(D/8) + sb)*64 + l)*8 + e).
+// split = 2: only bf16, pieces p = 0, 1, 2 of M (a0 = bf16(M), a1 = bf16(M -
+// a0), a2 = bf16(M - a0 - a1), residuals in FP64) in the permuted K order,
+// value a_p[32h + l%32][2(8sb + e) + l/32] at (((p*HALVES + h)*(D/8) + sb)*64
+// + l)*8 + e: 1.5x the floats of M_hi, the size of the split = 1 layout.
 static uint16_t to_bf16(float f)
 {
     uint32_t x;
@@ -176,6 +242,14 @@ static uint16_t to_bf16(float f)
     const uint32_t lsb = (x >> 16) & 1;          // round to nearest even
     x += 0x7FFFu + lsb;
     return (uint16_t)(x >> 16);
+}
+
+static double from_bf16(uint16_t b)
+{
+    const uint32_t x = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
 }
 
 void fused_operands(int kq, const double *u, int split, float *out)
@@ -187,6 +261,22 @@ void fused_operands(int kq, const double *u, int split, float *out)
         if ((i & 1) == 0) return (j & 1) == 0 ? ur : -ui;
         return (j & 1) == 0 ? ui : ur;
     };
+    if (split == 2) {
+        uint16_t *p = reinterpret_cast<uint16_t *>(out);
+        const int lob = d / 8;
+        for (int h = 0; h < halves; h++)
+            for (int sb = 0; sb < lob; sb++)
+                for (int l = 0; l < 64; l++)
+                    for (int e = 0; e < 8; e++) {
+                        double r = M(32 * h + (l & 31), 2 * (8 * sb + e) + (l >> 5));
+                        for (int q = 0; q < 3; q++) {
+                            const uint16_t b = to_bf16((float)r);
+                            p[(((q * halves + h) * lob + sb) * 64 + l) * 8 + e] = b;
+                            r -= from_bf16(b);
+                        }
+                    }
+        return;
+    }
     for (int h = 0; h < halves; h++)
         for (int s = 0; s < d; s++)
             for (int l = 0; l < 64; l++)
@@ -207,7 +297,9 @@ size_t fused_operand_floats(int kq, int split)
 {
     const int d = 1 << kq, halves = (2 * d) / 32;
     const size_t hi = (size_t)halves * d * 64;
-    return hi + (split ? hi / 2 : 0);            // bf16 lo part: half the bytes of the hi part
+    // split 1: the bf16 lo part is half the bytes of the hi part; split 2:
+    // three bf16 pieces, the same 1.5x
+    return hi + (split ? hi / 2 : 0);
 }
 
 template <int KQ, int TC>
@@ -215,8 +307,9 @@ static void launch_kq_tc(qsv_state *s, uint64_t ntiles, const FusedArgs &fa, con
 {
     float2 *a = (float2 *)s->amps;
     const dim3 g((unsigned)ntiles), t(2 << TC);
-    if (split) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, true>), g, t, 0, s->stream, a, ntiles, fa, mops);
-    else hipLaunchKernelGGL((k_fused_mfma<KQ, TC, false>), g, t, 0, s->stream, a, ntiles, fa, mops);
+    if (split == 2) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 2>), g, t, 0, s->stream, a, ntiles, fa, mops);
+    else if (split) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 1>), g, t, 0, s->stream, a, ntiles, fa, mops);
+    else hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 0>), g, t, 0, s->stream, a, ntiles, fa, mops);
 }
 
 // Launch one fused block (kq = 4 or 5, qubits[0..kq-1] distinct) with
@@ -227,6 +320,7 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused MFMA blocks need a c64 state");
     if (kq != 4 && kq != 5) return fail(QSV_ERR_ARG, "fused block size must be 4 or 5");
     if (tc < 6 || tc > 8) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
+    if (split < 0 || split > 2) return fail(QSV_ERR_ARG, "split must be 0, 1 or 2");
     if (s->n < kq + tc) return fail(QSV_ERR_ARG, "state too small for a fused tile");
     FusedArgs fa{};
     uint64_t qmask = 0;

@@ -11,8 +11,9 @@
 //   apply  for each block in order: its 2^(TB-k) groups are the columns; each
 //          wave takes 32-column chunks and runs the same MFMA product as the
 //          single-block kernel (FP32 v_mfma_f32_32x32x2f32 for M_hi, BF16
-//          v_mfma_f32_32x32x16_bf16 for the M_lo correction), writing the
-//          results back in place; a barrier separates blocks;
+//          v_mfma_f32_32x32x16_bf16 for the M_lo correction; or the BF16x6
+//          product on the BF16 MFMA alone), writing the results back in
+//          place; a barrier separates blocks;
 //   store  16-byte stores to the same addresses.
 // HBM traffic is one read and one write of the state per pass instead of per
 // block; MFMA work is the sum over the blocks.  (A persistent grid-stride form
@@ -43,16 +44,61 @@ struct PassArgs {
     PassBlock b[kMaxPassBlocks];
 };
 
-template <int KQ, bool SPLIT>
+// acc[h] += M B with the BF16x6 product (see qsv_fused.hip): B split exactly
+// into three bf16 pieces, the six piece products with i + j <= 2, smallest first.
+template <int HALVES, int LOBLK>
+__device__ __forceinline__ void pass_bf16x6(floatx16 (&acc)[HALVES], const bf16x8 (&ap)[3][HALVES][LOBLK],
+                                            const float *bv)
+{
+#pragma unroll
+    for (int sb = 0; sb < LOBLK; sb++) {
+        bf16x8 b0, b1, b2;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float x = bv[8 * sb + e];
+            const __bf16 t0 = (__bf16)x;
+            const float r1 = x - (float)t0;
+            const __bf16 t1 = (__bf16)r1;
+            b0[e] = t0;
+            b1[e] = t1;
+            b2[e] = (__bf16)(r1 - (float)t1);
+        }
+#pragma unroll
+        for (int h = 0; h < HALVES; h++) {
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2][h][sb], b0, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][h][sb], b1, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][h][sb], b2, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[1][h][sb], b0, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][h][sb], b1, acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[0][h][sb], b0, acc[h], 0, 0, 0);
+        }
+    }
+}
+
+// MODE: 0 FP32 product, 1 FP32 + BF16 lo correction, 2 BF16x6.
+template <int KQ, int MODE>
 __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlock &pb, int lane, int wave)
 {
     constexpr int D = 1 << KQ, HALVES = (2 * D) / 32, KSTEPS = D, LOBLK = D / 8;
-    float ahi[HALVES][KSTEPS];
+    constexpr bool SPLIT = MODE == 1, X6 = MODE == 2;
+    float ahi[X6 ? 1 : HALVES][X6 ? 1 : KSTEPS];
     bf16x8 alo[SPLIT ? HALVES : 1][SPLIT ? LOBLK : 1];
+    bf16x8 ap[X6 ? 3 : 1][X6 ? HALVES : 1][X6 ? LOBLK : 1];
+    if (X6) {
+        const bf16x8 *p = reinterpret_cast<const bf16x8 *>(pb.mops);
 #pragma unroll
-    for (int h = 0; h < HALVES; h++)
+        for (int q = 0; q < 3; q++)
 #pragma unroll
-        for (int s = 0; s < KSTEPS; s++) ahi[h][s] = pb.mops[(h * KSTEPS + s) * 64 + lane];
+            for (int h = 0; h < HALVES; h++)
+#pragma unroll
+                for (int sb = 0; sb < LOBLK; sb++)
+                    ap[X6 ? q : 0][X6 ? h : 0][X6 ? sb : 0] = p[((q * HALVES + h) * LOBLK + sb) * 64 + lane];
+    } else {
+#pragma unroll
+        for (int h = 0; h < HALVES; h++)
+#pragma unroll
+            for (int s = 0; s < KSTEPS; s++) ahi[X6 ? 0 : h][X6 ? 0 : s] = pb.mops[(h * KSTEPS + s) * 64 + lane];
+    }
     if (SPLIT) {
         const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(pb.mops + HALVES * KSTEPS * 64);
 #pragma unroll
@@ -75,10 +121,15 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
         float bv[KSTEPS];
 #pragma unroll
         for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[pb.erow[s] | ecol];
+        if constexpr (X6) {
+            pass_bf16x6<HALVES, LOBLK>(acc, ap, bv);
+        } else {
 #pragma unroll
-        for (int s = 0; s < KSTEPS; s++)
+            for (int s = 0; s < KSTEPS; s++)
 #pragma unroll
-            for (int h = 0; h < HALVES; h++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[h][s], bv[s], acc[h], 0, 0, 0);
+                for (int h = 0; h < HALVES; h++)
+                    acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[X6 ? 0 : h][X6 ? 0 : s], bv[s], acc[h], 0, 0, 0);
+        }
         if (SPLIT) {
             // M_lo in the permuted K order of the operand layout: the B values
             // are this lane's FP32 B values of K-steps 8 sb .. 8 sb + 7
@@ -106,31 +157,41 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
 
 // k = 4 operands of one block in registers (A of the FP32 product and, with
 // the split, A of the BF16 correction).
-template <bool SPLIT>
+template <int MODE>
 struct Ops4 {
-    float hi[16];
-    bf16x8 lo[SPLIT ? 2 : 1];
+    float hi[MODE == 2 ? 1 : 16];
+    bf16x8 lo[MODE == 1 ? 2 : 1];
+    bf16x8 p[MODE == 2 ? 3 : 1][1][MODE == 2 ? 2 : 1];   // BF16x6 pieces [piece][half][K-block]
 };
 
-template <bool SPLIT>
-__device__ __forceinline__ void load_ops4(Ops4<SPLIT> &o, const float *mops, int lane)
+template <int MODE>
+__device__ __forceinline__ void load_ops4(Ops4<MODE> &o, const float *mops, int lane)
 {
+    if (MODE == 2) {
+        const bf16x8 *p = reinterpret_cast<const bf16x8 *>(mops);
 #pragma unroll
-    for (int s = 0; s < 16; s++) o.hi[s] = mops[s * 64 + lane];
-    if (SPLIT) {
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int sb = 0; sb < 2; sb++) o.p[MODE == 2 ? q : 0][0][MODE == 2 ? sb : 0] = p[(q * 2 + sb) * 64 + lane];
+        return;
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) o.hi[MODE == 2 ? 0 : s] = mops[s * 64 + lane];
+    if (MODE == 1) {
         const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(mops + 16 * 64);
 #pragma unroll
-        for (int sb = 0; sb < 2; sb++) o.lo[SPLIT ? sb : 0] = lo[sb * 64 + lane];
+        for (int sb = 0; sb < 2; sb++) o.lo[MODE == 1 ? sb : 0] = lo[sb * 64 + lane];
     }
 }
 
 // One k = 4 block over the tile: 2^(TB-4) columns in 32-column chunks, chunk
 // c on wave c % 4.  (Interleaving two chunks' MFMA chains and prefetching the
 // next block's operands during this one measured within 3% of this form.)
-template <int TB, bool SPLIT>
-__device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBlock &pb, const Ops4<SPLIT> &o,
+template <int TB, int MODE>
+__device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBlock &pb, const Ops4<MODE> &o,
                                             int lane, int wave)
 {
+    constexpr bool SPLIT = MODE == 1;
     constexpr int NCB = TB - 4;                      // column bits
     constexpr int PER = ((1 << NCB) >> 5) / 4;       // chunks per wave: 1, 2, 4
     const float *const bplane = (lane >> 5) ? pim : pre;
@@ -146,8 +207,15 @@ __device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBl
         float bv[16];
 #pragma unroll
         for (int s = 0; s < 16; s++) bv[s] = bplane[pb.erow[s] | ecol];
+        if constexpr (MODE == 2) {
+            floatx16 a1[1] = {acc};
+            pass_bf16x6<1, 2>(a1, o.p, bv);
+            acc = a1[0];
+        } else {
 #pragma unroll
-        for (int s = 0; s < 16; s++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(o.hi[s], bv[s], acc, 0, 0, 0);
+            for (int s = 0; s < 16; s++)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(o.hi[MODE == 2 ? 0 : s], bv[s], acc, 0, 0, 0);
+        }
         if (SPLIT) {
 #pragma unroll
             for (int sb = 0; sb < 2; sb++) {
@@ -169,7 +237,7 @@ __device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBl
 
 // ANY5: some block of the pass has 5 qubits (otherwise only the k = 4 product
 // is compiled in, which keeps the register count at the k = 4 level).
-template <int TB, bool SPLIT, bool ANY5>
+template <int TB, int MODE, bool ANY5>
 __global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
 {
     constexpr int TSIZE = 1 << TB;
@@ -199,15 +267,15 @@ __global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint
     __syncthreads();
     if (ANY5) {
         for (int g = 0; g < pa.nblocks; g++) {
-            if (pa.b[g].kq == 4) pass_apply<4, SPLIT>(pre, pim, pa.b[g], lane, wave);
-            else pass_apply<5, SPLIT>(pre, pim, pa.b[g], lane, wave);
+            if (pa.b[g].kq == 4) pass_apply<4, MODE>(pre, pim, pa.b[g], lane, wave);
+            else pass_apply<5, MODE>(pre, pim, pa.b[g], lane, wave);
             __syncthreads();
         }
     } else {
         for (int g = 0; g < pa.nblocks; g++) {
-            Ops4<SPLIT> ops;
-            load_ops4<SPLIT>(ops, pa.b[g].mops, lane);
-            pass_apply4<TB, SPLIT>(pre, pim, pa.b[g], ops, lane, wave);
+            Ops4<MODE> ops;
+            load_ops4<MODE>(ops, pa.b[g].mops, lane);
+            pass_apply4<TB, MODE>(pre, pim, pa.b[g], ops, lane, wave);
             __syncthreads();
         }
     }
@@ -237,8 +305,10 @@ template <int TB, bool ANY5>
 static void launch_tb5(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split)
 {
     float2 *a = (float2 *)s->amps;
-    if (split) hipLaunchKernelGGL((k_fused_pass<TB, true, ANY5>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, pa);
-    else hipLaunchKernelGGL((k_fused_pass<TB, false, ANY5>), dim3((unsigned)ntiles), dim3(256), 0, s->stream, a, ntiles, pa);
+    const dim3 g((unsigned)ntiles), t(256);
+    if (split == 2) hipLaunchKernelGGL((k_fused_pass<TB, 2, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
+    else if (split) hipLaunchKernelGGL((k_fused_pass<TB, 1, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
+    else hipLaunchKernelGGL((k_fused_pass<TB, 0, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
 }
 
 template <int TB>
@@ -258,6 +328,7 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     const int G = (int)qsets.size();
     if (G < 1 || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..8 blocks per pass");
     if (tb < 11 || tb > kMaxPassTileBits || tb > s->n) return fail(QSV_ERR_ARG, "pass tile bits must be 11..13");
+    if (split < 0 || split > 2) return fail(QSV_ERR_ARG, "split must be 0, 1 or 2");
     uint64_t u = 0;
     for (auto *q : qsets) {
         if (q->size() != 4 && q->size() != 5) return fail(QSV_ERR_ARG, "pass blocks must have 4 or 5 qubits");

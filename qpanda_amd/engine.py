@@ -16,10 +16,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqsv.so")
 
 C128, C64 = 0, 1
+FILE_BINARY, FILE_TEXT = 0, 1
 
 EXPORTED = [
     "qsv_create", "qsv_destroy", "qsv_reset_zero", "qsv_num_qubits", "qsv_precision",
-    "qsv_set_state", "qsv_get_state", "qsv_device_ptr", "qsv_stream", "qsv_sync",
+    "qsv_set_state", "qsv_get_state", "qsv_save_state", "qsv_load_state", "qsv_device_ptr", "qsv_stream", "qsv_sync",
     "qsv_h", "qsv_x", "qsv_apply_1q", "qsv_diag", "qsv_rx", "qsv_ry", "qsv_rz",
     "qsv_cphase", "qsv_cr", "qsv_iswap", "qsv_sqiswap", "qsv_cswap", "qsv_apply_2q",
     "qsv_apply_block", "qsv_prob0", "qsv_norm2", "qsv_collapse", "qsv_measure",
@@ -86,6 +87,8 @@ def lib():
             "qsv_precision": ([vp], I),
             "qsv_set_state": ([vp, P], I),
             "qsv_get_state": ([vp, P], I),
+            "qsv_save_state": ([vp, C.c_char_p, I, I], I),
+            "qsv_load_state": ([vp, C.c_char_p, I], I),
             "qsv_device_ptr": ([vp, C.POINTER(vp)], I),
             "qsv_stream": ([vp, C.POINTER(vp)], I),
             "qsv_sync": ([vp], I),
@@ -213,6 +216,14 @@ class StateVector:
         _check(lib().qsv_get_state(self.h, out))
         return out.view(np.complex128)
 
+    def save(self, path: str, text: bool = False, digits: int = 0):
+        """State file: binary {f64 re, f64 im} records, or the getQState text
+        with `digits` significant digits (0: 6, as the reference prints)."""
+        _check(lib().qsv_save_state(self.h, path.encode(), FILE_TEXT if text else FILE_BINARY, digits))
+
+    def load(self, path: str, text: bool = False):
+        _check(lib().qsv_load_state(self.h, path.encode(), FILE_TEXT if text else FILE_BINARY))
+
     def sync(self):
         _check(lib().qsv_sync(self.h))
 
@@ -268,7 +279,7 @@ class StateVector:
     def cswap(self, c, a, b): _check(lib().qsv_cswap(self.h, c, a, b))
     def u2(self, qa, qb, u): _check(lib().qsv_apply_2q(self.h, qa, qb, _mat(u)))
 
-    def block(self, qubits, u, split=True):
+    def block(self, qubits, u, split=2):
         _check(lib().qsv_apply_block(self.h, _ints(qubits), len(qubits), _mat(u), int(split)))
 
     # ---- readout ----
@@ -327,7 +338,7 @@ class StateVector:
         _check(fn(self.h, _ints(qubits), len(qubits), gate_draws, C.byref(rs), shots, out, C.byref(ties)))
         return out[:shots], rs.value, ties.value
 
-    def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=True, scheduler=0,
+    def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=2, scheduler=0,
             rng_state=None, outcomes=None, profile=False, tile=0, group=0, pass_tb=0) -> RunStats:
         """scheduler: 0 greedy blocks (one sweep each), 1 learned (needs an attached
         cost model), 2 greedy blocks grouped into multi-block sweeps of <= group."""

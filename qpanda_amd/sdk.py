@@ -33,6 +33,7 @@ def lib():
             "qpanda_sdk_run": ([vp, I], I), "qpanda_sdk_last_run_sampled": ([vp], I),
             "qpanda_sdk_get_result": ([vp, cp, ip], I), "qpanda_sdk_get_state": ([vp, cp, ip], I),
             "qpanda_sdk_get_amplitudes": ([vp, C.c_void_p, C.c_int64], C.c_int64),
+            "qpanda_sdk_save_state": ([vp, cp, I], I),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -129,6 +130,10 @@ class SDK:
 
     def state_text(self) -> str:
         return _string(lib().qpanda_sdk_get_state, self.h)
+
+    def save_state(self, path: str, binary: bool) -> int:
+        """path.fullstate (state text) or path.fullstate.binary ({f64, f64} records)."""
+        return lib().qpanda_sdk_save_state(self.h, path.encode(), int(binary))
 
     def amplitudes(self):
         """complex128 numpy array of the last run's final state."""

@@ -5,7 +5,8 @@
 
 1. Collect: seeded circuits (random layers, QFT, hardware-efficient ansatz) at
    several qubit counts run on the fused path under every decision the
-   scheduler can make -- block size k = 2..5, split on / off, single-block
+   scheduler can make -- block size k = 2..5, the three products (FP32, FP32
+   + BF16 lo correction, BF16x6), single-block
    launches at every tile shape (6, 7, 8 column bits), and multi-block sweeps of
    up to 2, 3, 4 or 8 blocks at minimal / 12 / 13 tile bits -- with per-launch
    event timing; the engine's timing log gives one (features, ms) record per
@@ -30,9 +31,9 @@ import numpy as np
 from . import circuits as C
 from . import engine as E
 
-SINGLE = [dict(fuse_k=k, split=s, tile=t, scheduler=0) for k in (2, 3, 4, 5) for s in (False, True)
+SINGLE = [dict(fuse_k=k, split=s, tile=t, scheduler=0) for k in (2, 3, 4, 5) for s in (0, 1, 2)
           for t in (6, 7, 8)]
-PASSES = [dict(fuse_k=k, split=s, scheduler=2, group=g, pass_tb=tb) for k in (3, 4, 5) for s in (False, True)
+PASSES = [dict(fuse_k=k, split=s, scheduler=2, group=g, pass_tb=tb) for k in (3, 4, 5) for s in (0, 1, 2)
           for g in (2, 3, 4, 8) for tb in (0, 12, 13)]
 
 
@@ -84,7 +85,7 @@ METHODS = (("greedy_k4", dict(fuse_k=4, scheduler=0)),
            ("learned", dict(fuse_k=4, scheduler=1)))
 
 
-def compare(m, circ, reps=3, split=True):
+def compare(m, circ, reps=3, split=2):
     """Median device ms of each method on one circuit, plus fidelities to greedy."""
     n = circ.n
     res = {}
@@ -111,14 +112,14 @@ def compare(m, circ, reps=3, split=True):
     return res
 
 
-def evaluate(m, ns=(26, 28, 30), seed0=1000, depth=24, log=print):
+def evaluate(m, ns=(26, 28, 30), seed0=1000, depth=24, split=2, log=print):
     rows = []
     for n in ns:
         circs = [C.random_layers(n, depth, seed0 + n, haar_frac=0.1), C.hea(n, 6, seed0 + 50 + n)]
         if n <= 28:
             circs.append(C.qft(n))
         for circ in circs:
-            res = compare(m, circ)
+            res = compare(m, circ, split=split)
             row = {"circuit": circ.name, "n": n, "gates": circ.ngates, **{
                 f"{k}_{f}": v[f] for k, v in res.items() for f in ("ms", "launches", "blocks")},
                 "learned_predicted_ms": res["learned"]["predicted_ms"],
@@ -138,6 +139,7 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--split", type=int, default=2, help="product of the held-out comparison (0, 1, 2)")
     a = ap.parse_args(argv)
 
     def log(*s):
@@ -155,8 +157,8 @@ def main(argv=None):
     val_rmse = float(np.sqrt(np.mean((yv - t[va]) ** 2)))
     log(f"validation rmse (log ms): {val_rmse:.4f}  (~{100 * (math.exp(val_rmse) - 1):.1f}% relative)")
     m.save(a.out)
-    rows = evaluate(m, ns=(24, 28) if a.quick else (26, 28, 30), log=log)
-    report = {"records": int(x.shape[0]), "train_loss_initial": loss0, "train_loss": loss,
+    rows = evaluate(m, ns=(24, 28) if a.quick else (26, 28, 30), split=a.split, log=log)
+    report = {"records": int(x.shape[0]), "heldout_split": a.split, "train_loss_initial": loss0, "train_loss": loss,
               "val_rmse_log": val_rmse, "seconds": time.time() - t0, "heldout": rows,
               "geomean_gain_vs_greedy": float(np.exp(np.mean([np.log(r["gain_vs_greedy"]) for r in rows]))),
               "geomean_gain_vs_passes": float(np.exp(np.mean([np.log(r["gain_vs_passes"]) for r in rows]))),

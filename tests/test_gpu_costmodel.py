@@ -106,7 +106,8 @@ def test_learned_schedule_does_not_change_results():
 
 
 @pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
-def test_trained_model_gain_over_greedy():
+@pytest.mark.parametrize("split", [1, 2])
+def test_trained_model_gain_over_greedy(split):
     m = E.CostModel.load()
     rows = []
     for circ in (C.random_layers(28, 24, 4242, haar_frac=0.1), C.hea(28, 6, 77), C.qft(26)):
@@ -116,9 +117,9 @@ def test_trained_model_gain_over_greedy():
             tl, tg = [], []
             for _ in range(3):
                 a.reset()
-                tl.append(a.run(circ, fuse_k=4, scheduler=1).device_ms)
+                tl.append(a.run(circ, fuse_k=4, split=split, scheduler=1).device_ms)
                 b.reset()
-                tg.append(b.run(circ, fuse_k=4, scheduler=0).device_ms)
+                tg.append(b.run(circ, fuse_k=4, split=split, scheduler=0).device_ms)
             fid = a.fidelity(b)
         rows.append((circ.name, np.median(tg), np.median(tl), fid))
         say(f"{circ.name}: greedy k=4 {np.median(tg):.1f} ms, learned {np.median(tl):.1f} ms, "

@@ -1,4 +1,6 @@
-"""Fused FP32 MFMA blocks (complex64 state) against the FP64 oracle.
+"""Fused FP32 MFMA blocks (complex64 state) against the FP64 oracle, for the
+three products: FP32 (split 0), FP32 + BF16 lo correction (split 1) and BF16x6
+(split 2, the product on the BF16 matrix cores).
 
 Bars: one block application within 2e-6 max amplitude error (FP32 storage and
 accumulation, ~1e-7 relative); whole fused circuits at fidelity >= 1 - 1e-6
@@ -33,7 +35,7 @@ QSETS = {
 
 
 @pytest.mark.parametrize("k", [2, 3, 4, 5])
-@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("split", [0, 1, 2])
 def test_block_mfma_vs_oracle(k, split):
     n = 16
     for idx, qs in enumerate(QSETS[k]):
@@ -63,24 +65,26 @@ def test_split_is_not_less_accurate():
     for _ in range(20):
         st.block(qs, u)
     errs = {}
-    for split in (False, True):
+    for split in (0, 1, 2):
         with E.StateVector(n, E.C64) as sv:
             sv.set(psi)
             for _ in range(20):
                 sv.block(qs, u, split=split)
             errs[split] = np.abs(sv.get() - st.psi).max()
-    say("20 blocks, max error without / with split", errs[False], errs[True])
-    assert errs[True] <= errs[False] * 1.1
+    say("20 blocks, max error FP32 / split / BF16x6", errs[0], errs[1], errs[2])
+    assert errs[1] <= errs[0] * 1.1
+    assert errs[2] <= errs[0] * 1.1
 
 
 @pytest.mark.parametrize("k", [2, 3, 4, 5])
-def test_fused_circuit_fidelity(k):
+@pytest.mark.parametrize("split", [1, 2])
+def test_fused_circuit_fidelity(k, split):
     n = 16
     circ = C.random_layers(n, 20, 500 + k, haar_frac=0.1)
     ref, _ = O.run_circuit(circ)
-    say("fused circuit k", k)
+    say("fused circuit k", k, "split", split)
     with E.StateVector(n, E.C64) as sv:
-        st = sv.run(circ, fuse_k=k, split=True)
+        st = sv.run(circ, fuse_k=k, split=split)
         got = sv.get()
     assert st.blocks > 0 and st.blocks < circ.ngates
     fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
@@ -139,16 +143,17 @@ def test_fused_large_state_sweep_30q():
 
 @pytest.mark.parametrize("group,pass_tb", [(2, 0), (3, 0), (8, 0), (4, 13)])
 @pytest.mark.parametrize("k", [4, 5])
-def test_multi_block_passes_vs_oracle(group, pass_tb, k):
+@pytest.mark.parametrize("split", [1, 2])
+def test_multi_block_passes_vs_oracle(group, pass_tb, k, split):
     n = 18
     circ = C.random_layers(n, 16, 900 + group + k, haar_frac=0.1)
     ref, _ = O.run_circuit(circ)
-    say("passes group", group, "tb", pass_tb, "k", k)
+    say("passes group", group, "tb", pass_tb, "k", k, "split", split)
     with E.StateVector(n, E.C64) as sv:
-        st = sv.run(circ, fuse_k=k, split=True, scheduler=2, group=group, pass_tb=pass_tb, profile=True)
+        st = sv.run(circ, fuse_k=k, split=split, scheduler=2, group=group, pass_tb=pass_tb, profile=True)
         got = sv.get()
         single = E.StateVector(n, E.C64)
-        st1 = single.run(circ, fuse_k=k, split=True, profile=True)
+        st1 = single.run(circ, fuse_k=k, split=split, profile=True)
         single.close()
     cl = st.classes()
     sweeps = cl.get("fused", {}).get("launches", 0) + cl.get("pass", {}).get("launches", 0)
