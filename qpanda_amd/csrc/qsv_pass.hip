@@ -15,10 +15,21 @@
 //          product on the BF16 MFMA alone), writing the results back in
 //          place; a barrier separates blocks;
 //   store  16-byte stores to the same addresses.
+// LDS layout: tile element e sits at float index P(e) = e ^ S(e) of its
+// plane, S XOR-ing bank bits 1..4 with a fixed linear function of bits 5..8
+// (kLdsSwizzle).  Without it, the 32 columns a half-wave reads or writes land
+// 16 floats apart when the block holds the tile's lowest qubits (a 16-way bank
+// conflict) and 2-8 ways for most other blocks; the constants were chosen over
+// the blocks of random, QFT and HEA circuits (average bank cycles per access
+// 2.7-3.6 without, 1.2-1.4 with).  P is linear over GF(2) and leaves bit 0
+// alone, so a position is still the XOR of precomputed row and column parts
+// and (re, im) pair writes stay 8-byte aligned.
 // HBM traffic is one read and one write of the state per pass instead of per
 // block; MFMA work is the sum over the blocks.  (A persistent grid-stride form
 // that prefetches the next tile into registers measured ~10% slower at 30
-// qubits than one workgroup per tile.)  Whether a grouping pays is a
+// qubits than one workgroup per tile with the FP32 product, and 4% slower with
+// BF16x6 on 64 KiB tiles, where it costs no occupancy; capping registers for
+// five resident 32 KiB tiles per CU measured 4% slower as well.)  Whether a grouping pays is a
 // scheduling decision (greedy grouping, or the cost model in qsv_costmodel.hip).
 #include "qsv_internal.h"
 #include "qsv_sched.h"
@@ -28,12 +39,24 @@ namespace qsv {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
+// Bank bits 1..4 XOR-ed into the LDS float index by bits 5, 6, 7, 8.
+constexpr int kLdsSwizzle[4] = {0x1E, 0x12, 0x14, 0x1C};
+
+__host__ __device__ inline int lds_swizzle(int e)
+{
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (e >> (5 + i) & 1) s ^= kLdsSwizzle[i];
+    return e ^ s;
+}
+
 struct PassBlock {
     const float *mops;      // operand-ready matrix (hi floats, then bf16 lo)
     int kq;                 // 4 or 5
     int ncolbits;           // TB - kq
-    int erow[32];           // compact tile position of row (amplitude) r
-    int col_rank[10];       // rank in L of the i-th column bit
+    int erow[32];           // swizzled LDS index of row (amplitude) r
+    int colp[10];           // swizzled LDS index of the i-th column bit alone
 };
 
 struct PassArgs {
@@ -44,6 +67,27 @@ struct PassArgs {
     PassBlock b[kMaxPassBlocks];
 };
 
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Exact three-piece bf16 split of two FP32 values, one packed conversion per
+// piece pair: b = b0 + b1 + b2 (each residual is exact in FP32 by Sterbenz's
+// lemma and the last has at most 8 significant bits).  The bf16 pairs come
+// back as the 32-bit words the MFMA operand registers hold.
+__device__ __forceinline__ void split3_pair(float x0, float x1, uint32_t &p0, uint32_t &p1, uint32_t &p2)
+{
+    auto pk = [](float a, float b) {
+        const bf16x2 v = __builtin_convertvector((float2v){a, b}, bf16x2);
+        return __builtin_bit_cast(uint32_t, v);
+    };
+    auto lo = [](uint32_t w) { return __builtin_bit_cast(float, w << 16); };
+    auto hi = [](uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); };
+    p0 = pk(x0, x1);
+    const float r0 = x0 - lo(p0), r1 = x1 - hi(p0);
+    p1 = pk(r0, r1);
+    p2 = pk(r0 - lo(p1), r1 - hi(p1));
+}
+
 // acc[h] += M B with the BF16x6 product (see qsv_fused.hip): B split exactly
 // into three bf16 pieces, the six piece products with i + j <= 2, smallest first.
 template <int HALVES, int LOBLK>
@@ -52,17 +96,12 @@ __device__ __forceinline__ void pass_bf16x6(floatx16 (&acc)[HALVES], const bf16x
 {
 #pragma unroll
     for (int sb = 0; sb < LOBLK; sb++) {
-        bf16x8 b0, b1, b2;
+        uint32_t w0[4], w1[4], w2[4];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const float x = bv[8 * sb + e];
-            const __bf16 t0 = (__bf16)x;
-            const float r1 = x - (float)t0;
-            const __bf16 t1 = (__bf16)r1;
-            b0[e] = t0;
-            b1[e] = t1;
-            b2[e] = (__bf16)(r1 - (float)t1);
-        }
+        for (int e = 0; e < 4; e++) split3_pair(bv[8 * sb + 2 * e], bv[8 * sb + 2 * e + 1], w0[e], w1[e], w2[e]);
+        const bf16x8 b0 = __builtin_bit_cast(bf16x8, w0);
+        const bf16x8 b1 = __builtin_bit_cast(bf16x8, w1);
+        const bf16x8 b2 = __builtin_bit_cast(bf16x8, w2);
 #pragma unroll
         for (int h = 0; h < HALVES; h++) {
             acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ap[2][h][sb], b0, acc[h], 0, 0, 0);
@@ -112,7 +151,7 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
         const int col = chunk * 32 + (lane & 31);
         int ecol = 0;
         for (int i = 0; i < pb.ncolbits; i++)
-            if (col >> i & 1) ecol |= 1 << pb.col_rank[i];
+            if (col >> i & 1) ecol ^= pb.colp[i];
         floatx16 acc[HALVES];
 #pragma unroll
         for (int h = 0; h < HALVES; h++)
@@ -120,7 +159,7 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
             for (int v = 0; v < 16; v++) acc[h][v] = 0.0f;
         float bv[KSTEPS];
 #pragma unroll
-        for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[pb.erow[s] | ecol];
+        for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[pb.erow[s] ^ ecol];
         if constexpr (X6) {
             pass_bf16x6<HALVES, LOBLK>(acc, ap, bv);
         } else {
@@ -148,7 +187,7 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
 #pragma unroll
             for (int v = 0; v < 16; v++) {
                 const int m = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const int e = pb.erow[m >> 1] | ecol;
+                const int e = pb.erow[m >> 1] ^ ecol;
                 if (m & 1) pim[e] = acc[h][v];
                 else pre[e] = acc[h][v];
             }
@@ -187,26 +226,48 @@ __device__ __forceinline__ void load_ops4(Ops4<MODE> &o, const float *mops, int 
 // One k = 4 block over the tile: 2^(TB-4) columns in 32-column chunks, chunk
 // c on wave c % 4.  (Interleaving two chunks' MFMA chains and prefetching the
 // next block's operands during this one measured within 3% of this form.)
+// Addressing: column bits 0..4 come from the lane (their tile position is
+// computed once per block), bits 5.. from the wave-uniform chunk index (scalar
+// instructions); the 16 row addresses of a chunk serve the B reads and the
+// write-back (a lane writes rows 4j' + 2 hf + {0, 1} of both planes).
 template <int TB, int MODE>
-__device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBlock &pb, const Ops4<MODE> &o,
-                                            int lane, int wave)
+__device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, const Ops4<MODE> &o, int lane,
+                                            int wave)
 {
     constexpr bool SPLIT = MODE == 1;
     constexpr int NCB = TB - 4;                      // column bits
     constexpr int PER = ((1 << NCB) >> 5) / 4;       // chunks per wave: 1, 2, 4
-    const float *const bplane = (lane >> 5) ? pim : pre;
-    for (int c = 0; c < PER; c++) {
-        const int col = (wave + 4 * c) * 32 + (lane & 31);
-        int ecol = 0;
+    constexpr int PLANE = 1 << TB;                   // im plane offset (floats)
+    const int hf = lane >> 5;
+    // byte offsets: the swizzled index is linear, so an element's offset is
+    // the XOR of its row part, its column bits' parts and the plane bit
+    int elane = hf << (TB + 2);
 #pragma unroll
-        for (int i = 0; i < NCB; i++)
-            if (col >> i & 1) ecol |= 1 << pb.col_rank[i];
+    for (int i = 0; i < 5; i++)
+        if (lane >> i & 1) elane ^= 4 * pb.colp[i];
+    // rows this lane writes: 4 (j >> 1) + (j & 1) + 2 hf (arithmetic, not a
+    // select between array entries, which would index the array dynamically)
+    int wrow[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int r0 = 4 * (j >> 1) + (j & 1);
+        wrow[j] = 4 * (pb.erow[r0] + hf * (pb.erow[r0 + 2] - pb.erow[r0]));
+    }
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    char *const base = reinterpret_cast<char *>(pre);
+    for (int c = 0; c < PER; c++) {
+        const int chunk = wv + 4 * c;                // wave-uniform
+        int ehi = 0;
+#pragma unroll
+        for (int i = 5; i < NCB; i++)
+            if (chunk >> (i - 5) & 1) ehi ^= 4 * pb.colp[i];
+        const int ecol = elane ^ ehi;                // includes this lane's plane
         floatx16 acc;
 #pragma unroll
         for (int v = 0; v < 16; v++) acc[v] = 0.0f;
         float bv[16];
 #pragma unroll
-        for (int s = 0; s < 16; s++) bv[s] = bplane[pb.erow[s] | ecol];
+        for (int s = 0; s < 16; s++) bv[s] = *reinterpret_cast<const float *>(base + ((4 * pb.erow[s]) ^ ecol));
         if constexpr (MODE == 2) {
             floatx16 a1[1] = {acc};
             pass_bf16x6<1, 2>(a1, o.p, bv);
@@ -225,12 +286,14 @@ __device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBl
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.lo[SPLIT ? sb : 0], bl, acc, 0, 0, 0);
             }
         }
+        // output register v holds real row m = 8 (v >> 2) + 4 hf + (v & 3):
+        // complex row 4 (v >> 2) + 2 hf + ((v & 3) >> 1), plane v & 1; the
+        // plane bit of ecol belongs to the B reads, so it is cleared here
+        const int wcol = ecol & ~(1 << (TB + 2));
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int m = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-            const int e = pb.erow[m >> 1] | ecol;
-            if (m & 1) pim[e] = acc[v];
-            else pre[e] = acc[v];
+            const int j = 2 * (v >> 2) + ((v & 3) >> 1);
+            *reinterpret_cast<float *>(base + (wrow[j] ^ wcol) + (v & 1) * 4 * PLANE) = acc[v];
         }
     }
 }
@@ -238,7 +301,7 @@ __device__ __forceinline__ void pass_apply4(float *pre, float *pim, const PassBl
 // ANY5: some block of the pass has 5 qubits (otherwise only the k = 4 product
 // is compiled in, which keeps the register count at the k = 4 level).
 template <int TB, int MODE, bool ANY5>
-__global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
+__device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t ntiles, const PassArgs &pa)
 {
     constexpr int TSIZE = 1 << TB;
     constexpr int NLD = TSIZE / 512;         // 16-byte loads per thread
@@ -258,9 +321,12 @@ __global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint
     float4 buf[NLD];
 #pragma unroll
     for (int i = 0; i < NLD; i++) buf[i] = *reinterpret_cast<const float4 *>(a + base + toff + pa.hioff[i]);
+    // elements 2 tid + 512 i: bits 5..8 come from tid alone, so the swizzle is
+    // one XOR per thread; it keeps bit 0, so element + 1 follows at index + 1
+    const int sw = lds_swizzle(2 * tid) ^ (2 * tid);
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
-        const int e = 2 * (tid + 256 * i);
+        const int e = (2 * (tid + 256 * i)) ^ sw;
         pre[e] = buf[i].x; pim[e] = buf[i].y;
         pre[e + 1] = buf[i].z; pim[e + 1] = buf[i].w;
     }
@@ -275,18 +341,24 @@ __global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint
         for (int g = 0; g < pa.nblocks; g++) {
             Ops4<MODE> ops;
             load_ops4<MODE>(ops, pa.b[g].mops, lane);
-            pass_apply4<TB, MODE>(pre, pim, pa.b[g], ops, lane, wave);
+            pass_apply4<TB, MODE>(pre, pa.b[g], ops, lane, wave);
             __syncthreads();
         }
     }
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
-        const int e = 2 * (tid + 256 * i);
+        const int e = (2 * (tid + 256 * i)) ^ sw;
         float4 v;
         v.x = pre[e]; v.y = pim[e];
         v.z = pre[e + 1]; v.w = pim[e + 1];
         *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
     }
+}
+
+template <int TB, int MODE, bool ANY5>
+__global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
+{
+    fused_pass_body<TB, MODE, ANY5>(a, ntiles, pa);
 }
 
 // Tile bits a pass over these (widened) blocks needs: qubits 0..3 (the
@@ -368,13 +440,13 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
             int e = 0;
             for (int j = 0; j < pb.kq; j++)
                 if (r >> j & 1) e |= 1 << rank(q[j]);
-            pb.erow[r] = e;
+            pb.erow[r] = lds_swizzle(e);
         }
         int c = 0;
         for (int i = 0; i < tb; i++) {
             bool inq = false;
             for (int x : q) inq |= (x == lpos[i]);
-            if (!inq) pb.col_rank[c++] = i;
+            if (!inq) pb.colp[c++] = lds_swizzle(1 << i);
         }
     }
     const uint64_t ntiles = s->dim >> tb;
