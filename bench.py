@@ -470,9 +470,10 @@ def main_batch(a):
     b.close()
     nq = np.asarray(nq)
     ly = np.asarray(ly)
-    # algorithmic HBM bytes: instances above 14 qubits stream their state twice per layer
-    big = nq > 14
-    hbm_bytes = float(np.sum(ly[big] * 2 * 2 * 8 * (2.0 ** nq[big])))
+    # algorithmic HBM bytes: instances above 14 qubits stream their state twice per layer,
+    # less the read of |0...0> before the first pass and the store after the last
+    big = (nq > 14) & (ly > 0)
+    hbm_bytes = float(np.sum((4 * ly[big] - 2) * 8 * (2.0 ** nq[big])))
     gate_apps = float(np.sum(ly * (2 * nq - 1)))
     result = {
         "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -486,7 +487,7 @@ def main_batch(a):
             "unit": "GB/s", "frac": result["hbm_gbs_streamed_instances"] / HBM_PEAK_GBS, "traffic": None,
             "kernel_class": "batch",
             "note": "instances of <= 14 qubits stay in LDS and move no HBM bytes; bytes count the two "
-                    "LDS-tiled passes per layer of the larger ones"}
+                    "LDS-tiled passes per layer of the larger ones ((4 L - 2) x 8 x 2^n per instance)"}
     cpu = None
     if not a.no_cpu:
         log("cpu baseline")

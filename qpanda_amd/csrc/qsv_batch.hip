@@ -7,14 +7,19 @@
 //
 // One persistent workgroup per CU pulls instances from an atomic counter; the
 // host orders instances by decreasing work, so the largest start first.
-//   n <= 14: the whole state lives in LDS (2^14 x 8 B = 128 KiB); every gate is
-//            an LDS sweep between barriers; HBM is never touched.
+//   n <= 14: the whole state lives in LDS (2^14 x 8 B = 128 KiB); a layer is
+//            ceil(n / 4) LDS sweeps (rotations in groups of four qubits, the
+//            CNOT ladder folded into the last sweep's write-back); HBM is never
+//            touched.
 //   n >  14: the state lives in a per-workgroup HBM slice; each layer is two
 //            LDS-tiled passes over it: qubits 0..13 (contiguous 2^14 tiles: the
 //            U's on qubits < 14 and CNOT(q, q+1) for q <= 12), then qubits
 //            13..n-1 plus the 27-n lowest (the U's on qubits >= 14 and the rest
 //            of the ladder, in order).  This is the circuit's own order up to
-//            commuting disjoint gates.
+//            commuting disjoint gates.  The first pass of layer 0 starts from
+//            |0...0> without reading it and the last pass of the last layer
+//            reduces instead of storing: (4 L - 2) x 8 x 2^n bytes of HBM
+//            traffic per instance.
 // The counter is reset by a one-thread kernel (no memset node), so the pair of
 // launches can be captured in a hipGraph and replayed.
 #include "qsv_internal.h"
@@ -38,43 +43,121 @@ struct Inst {
     int64_t index;                           // caller's instance index
 };
 
-__device__ __forceinline__ uint32_t ins0(uint32_t i, int r)
+
+__device__ __forceinline__ uint32_t ladder_perm(uint32_t x, int l0, uint32_t lmask)
 {
-    return ((i >> r) << (r + 1)) | (i & ((1u << r) - 1));
+    // CNOT(b, b + 1) for b = l0 .. l1 - 1 in order maps x to y with bits
+    // l0..l1 of y the prefix XOR of those of x
+    uint32_t f = (x >> l0) & lmask;
+    f ^= f << 1;
+    f ^= f << 2;
+    f ^= f << 4;
+    f ^= f << 8;
+    f &= lmask;
+    return (x & ~(lmask << l0)) | (f << l0);
 }
 
-// Apply U (8 floats, row-major complex) to tile bit r of an LDS tile of 2^tb.
-template <int T>
-__device__ __forceinline__ void lds_u(float2 *t, int tb, int r, const float *u)
+// One LDS sweep applying the rotations of the G contiguous tile bits gb ..
+// gb + G - 1 (matrices u[i]) to every group of 2^G amplitudes.  LAST: all
+// reads happen before all writes (a barrier between), and amplitude x is
+// written to ladder_perm(x) -- the CNOT ladder folded into the write-back;
+// a thread then holds 2^tb / T <= 2^kLdsBits / T amplitudes in registers.
+template <int T, int G, bool LAST>
+__device__ __forceinline__ void group_sweep(float2 *t, int tb, int gb, const float (&u)[4][8], int l0, uint32_t lmask)
 {
-    const uint32_t np = 1u << (tb - 1);
-    for (uint32_t p = threadIdx.x; p < np; p += T) {
-        const uint32_t i0 = ins0(p, r), i1 = i0 | (1u << r);
-        const float2 a = t[i0], b = t[i1];
-        float2 y0, y1;
-        y0.x = u[0] * a.x - u[1] * a.y + u[2] * b.x - u[3] * b.y;
-        y0.y = u[0] * a.y + u[1] * a.x + u[2] * b.y + u[3] * b.x;
-        y1.x = u[4] * a.x - u[5] * a.y + u[6] * b.x - u[7] * b.y;
-        y1.y = u[4] * a.y + u[5] * a.x + u[6] * b.y + u[7] * b.x;
-        t[i0] = y0;
-        t[i1] = y1;
+    constexpr int D = 1 << G;
+    constexpr int ITER = LAST ? (1 << kLdsBits) / T / D : 1;   // groups per thread, compile-time when LAST
+    const uint32_t ngroups = 1u << (tb - G);
+    float2 keep[LAST ? ITER * D : 1];
+    auto apply = [&](uint32_t p, float2 (&v)[D]) {
+        const uint32_t base = (p & ((1u << gb) - 1)) | ((p >> gb) << (gb + G));
+#pragma unroll
+        for (int j = 0; j < D; j++) v[j] = t[base | ((uint32_t)j << gb)];
+#pragma unroll
+        for (int i = 0; i < G; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                if (j >> i & 1) continue;
+                const float2 a = v[j], b = v[j | (1 << i)];
+                const float *m = u[i];
+                v[j].x = m[0] * a.x - m[1] * a.y + m[2] * b.x - m[3] * b.y;
+                v[j].y = m[0] * a.y + m[1] * a.x + m[2] * b.y + m[3] * b.x;
+                v[j | (1 << i)].x = m[4] * a.x - m[5] * a.y + m[6] * b.x - m[7] * b.y;
+                v[j | (1 << i)].y = m[4] * a.y + m[5] * a.x + m[6] * b.y + m[7] * b.x;
+            }
+        return base;
+    };
+    if (!LAST) {
+        for (uint32_t p = threadIdx.x; p < ngroups; p += T) {
+            float2 v[D];
+            const uint32_t base = apply(p, v);
+#pragma unroll
+            for (int j = 0; j < D; j++) t[base | ((uint32_t)j << gb)] = v[j];
+        }
+        __syncthreads();
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < ITER; k++) {
+        const uint32_t p = threadIdx.x + (uint32_t)k * T;
+        if (p < ngroups) {
+            float2 v[D];
+            apply(p, v);
+#pragma unroll
+            for (int j = 0; j < D; j++) keep[LAST ? k * D + j : 0] = v[j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITER; k++) {
+        const uint32_t p = threadIdx.x + (uint32_t)k * T;
+        if (p < ngroups) {
+            const uint32_t base = (p & ((1u << gb) - 1)) | ((p >> gb) << (gb + G));
+#pragma unroll
+            for (int j = 0; j < D; j++) t[ladder_perm(base | ((uint32_t)j << gb), l0, lmask)] = keep[LAST ? k * D + j : 0];
+        }
     }
     __syncthreads();
 }
 
-// CNOT with control tile bit rc and target tile bit rt.
-template <int T>
-__device__ __forceinline__ void lds_cnot(float2 *t, int tb, int rc, int rt)
+template <int T, bool LAST>
+__device__ __forceinline__ void group_sweep_g(float2 *t, int tb, int gb, int g, const float (&u)[4][8], int l0,
+                                              uint32_t lmask)
 {
-    const int lo = rc < rt ? rc : rt, hi = rc < rt ? rt : rc;
-    const uint32_t nq = 1u << (tb - 2);
-    for (uint32_t p = threadIdx.x; p < nq; p += T) {
-        const uint32_t i = ins0(ins0(p, lo), hi) | (1u << rc);
-        const float2 a = t[i], b = t[i | (1u << rt)];
-        t[i] = b;
-        t[i | (1u << rt)] = a;
+    switch (g) {
+    case 1: group_sweep<T, 1, LAST>(t, tb, gb, u, l0, lmask); break;
+    case 2: group_sweep<T, 2, LAST>(t, tb, gb, u, l0, lmask); break;
+    case 3: group_sweep<T, 3, LAST>(t, tb, gb, u, l0, lmask); break;
+    default: group_sweep<T, 4, LAST>(t, tb, gb, u, l0, lmask); break;
     }
-    __syncthreads();
+}
+
+// One layer's share on an LDS tile of 2^tb amplitudes: the rotations on the
+// contiguous tile bits [ub0, ub1) -- U for tile bit b at ubase + 8 (b + qoff)
+// -- then the CNOT ladder CNOT(b, b + 1), b = l0 .. l1 - 1, in order.  The
+// rotations commute (disjoint qubits), so they go in groups of up to four
+// bits, one LDS sweep per group instead of one per gate (the first group takes
+// the remainder, so the last is as wide as possible); the ladder is folded
+// into the last group's write-back.
+template <int T>
+__device__ __forceinline__ void tile_layer(float2 *t, int tb, int ub0, int ub1, const float *ubase, int qoff, int l0,
+                                           int l1)
+{
+    const uint32_t lmask = l1 > l0 ? ((1u << (l1 - l0 + 1)) - 1) : 0;
+    const int nu = ub1 - ub0;
+    int gb = ub0;
+    int g = nu % 4 ? nu % 4 : 4;
+    while (gb < ub1) {
+        float u[4][8];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) u[i][k] = i < g ? ubase[8 * (gb + i + qoff) + k] : 0.f;
+        if (gb + g >= ub1) group_sweep_g<T, true>(t, tb, gb, g, u, l0, lmask);
+        else group_sweep_g<T, false>(t, tb, gb, g, u, l0, lmask);
+        gb += g;
+        g = 4;
+    }
 }
 
 __device__ __forceinline__ double block_sum_d(double v, double *red)
@@ -110,7 +193,7 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
 {
     // exactly 128 KiB of LDS: the work-counter slot and the reduction scratch
     // reuse the tile while it holds no state (between instances)
-    __shared__ float2 tile[1 << kLdsBits];
+    __shared__ __align__(16) float2 tile[1 << kLdsBits];
     unsigned int *const next_slot = reinterpret_cast<unsigned int *>(tile);
     double *const red = reinterpret_cast<double *>(tile);
     float2 *slice = scratch + (size_t)blockIdx.x * ((size_t)1 << kMaxBatchQubits);
@@ -129,32 +212,50 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
             const uint32_t dim = 1u << n;
             for (uint32_t i = threadIdx.x; i < dim; i += T) tile[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
             __syncthreads();
-            for (int l = 0; l < I.layers; l++) {
-                for (int q = 0; q < n; q++) lds_u<T>(tile, n, q, U + ((size_t)l * n + q) * 8);
-                for (int q = 0; q + 1 < n; q++) lds_cnot<T>(tile, n, q, q + 1);
-            }
+            for (int l = 0; l < I.layers; l++) tile_layer<T>(tile, n, 0, n, U + (size_t)l * n * 8, 0, 0, n - 1);
             for (uint32_t i = threadIdx.x; i < dim; i += T) {
                 const double x = tile[i].x, y = tile[i].y;
                 acc += (x * x + y * y) * zz_weight(i, n);
             }
         } else {
+            // HBM slice; each tile moves as 16-byte accesses, all of a thread's
+            // loads issued before its LDS writes (and LDS reads before stores).
+            // Layer 0's first pass starts from |0...0> without reading it, and
+            // the last pass accumulates the ZZ sum instead of storing.
             const uint64_t dim = uint64_t(1) << n;
-            for (uint64_t i = threadIdx.x; i < dim; i += T) slice[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
-            __syncthreads();
-            const int c = 27 - n;                               // low bits in the high-qubit tile
-            const uint32_t ts = 1u << kLdsBits;
+            constexpr uint32_t TS = 1u << kLdsBits;
+            constexpr int NL = (int)TS / (2 * T);          // 16-byte accesses per thread per tile
+            const int c = 27 - n;                           // low bits in the high-qubit tile
+            if (I.layers == 0 && threadIdx.x == 0) acc = zz_weight(0, n);
             for (int l = 0; l < I.layers; l++) {
                 const float *Ul = U + (size_t)l * n * 8;
+                const bool last = l + 1 == I.layers;
                 // pass A: qubits 0..13, contiguous tiles
-                for (uint64_t base = 0; base < dim; base += ts) {
-                    for (uint32_t e = threadIdx.x; e < ts; e += T) tile[e] = slice[base + e];
+                for (uint64_t base = 0; base < dim; base += TS) {
+                    if (l == 0) {
+                        for (uint32_t e = threadIdx.x; e < TS; e += T)
+                            tile[e] = make_float2(base == 0 && e == 0 ? 1.f : 0.f, 0.f);
+                    } else {
+                        float4 buf[NL];
+#pragma unroll
+                        for (int i = 0; i < NL; i++)
+                            buf[i] = *reinterpret_cast<const float4 *>(slice + base + 2 * (threadIdx.x + T * i));
+#pragma unroll
+                        for (int i = 0; i < NL; i++)
+                            *reinterpret_cast<float4 *>(tile + 2 * (threadIdx.x + T * i)) = buf[i];
+                    }
                     __syncthreads();
-                    for (int q = 0; q < kLdsBits; q++) lds_u<T>(tile, kLdsBits, q, Ul + q * 8);
-                    for (int q = 0; q + 1 < kLdsBits; q++) lds_cnot<T>(tile, kLdsBits, q, q + 1);
-                    for (uint32_t e = threadIdx.x; e < ts; e += T) slice[base + e] = tile[e];
+                    tile_layer<T>(tile, kLdsBits, 0, kLdsBits, Ul, 0, 0, kLdsBits - 1);
+                    float4 out[NL];
+#pragma unroll
+                    for (int i = 0; i < NL; i++) out[i] = *reinterpret_cast<const float4 *>(tile + 2 * (threadIdx.x + T * i));
+#pragma unroll
+                    for (int i = 0; i < NL; i++)
+                        *reinterpret_cast<float4 *>(slice + base + 2 * (threadIdx.x + T * i)) = out[i];
                     __syncthreads();
                 }
-                // pass B: tile bits 0..c-1 = qubits 0..c-1, bits c.. = qubits 13..n-1
+                // pass B: tile bits 0..c-1 = qubits 0..c-1, bits c.. = qubits 13..n-1;
+                // c >= 5, so tile elements e, e + 1 (e even) are adjacent in the slice
                 const uint64_t ntile = dim >> kLdsBits;
                 for (uint64_t t = 0; t < ntile; t++) {
                     // t enumerates qubits c..12 (13 - c = n - 14 bits)
@@ -162,17 +263,35 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
                     auto gaddr = [&](uint32_t e) -> uint64_t {
                         return (uint64_t)(e & ((1u << c) - 1)) | fixed | ((uint64_t)(e >> c) << 13);
                     };
-                    for (uint32_t e = threadIdx.x; e < ts; e += T) tile[e] = slice[gaddr(e)];
+                    {
+                        float4 buf[NL];
+#pragma unroll
+                        for (int i = 0; i < NL; i++)
+                            buf[i] = *reinterpret_cast<const float4 *>(slice + gaddr(2 * (threadIdx.x + T * i)));
+#pragma unroll
+                        for (int i = 0; i < NL; i++)
+                            *reinterpret_cast<float4 *>(tile + 2 * (threadIdx.x + T * i)) = buf[i];
+                    }
                     __syncthreads();
-                    for (int q = kLdsBits; q < n; q++) lds_u<T>(tile, kLdsBits, c + (q - 13), Ul + q * 8);
-                    for (int q = 13; q + 1 < n; q++) lds_cnot<T>(tile, kLdsBits, c + (q - 13), c + (q + 1 - 13));
-                    for (uint32_t e = threadIdx.x; e < ts; e += T) slice[gaddr(e)] = tile[e];
+                    // qubit q at tile bit c + q - 13: rotations on qubits 14..n-1, ladder 13..n-1
+                    tile_layer<T>(tile, kLdsBits, c + 1, c + n - 13, Ul, 13 - c, c, c + n - 14);
+                    float4 out[NL];
+#pragma unroll
+                    for (int i = 0; i < NL; i++) out[i] = *reinterpret_cast<const float4 *>(tile + 2 * (threadIdx.x + T * i));
+                    if (last) {
+#pragma unroll
+                        for (int i = 0; i < NL; i++) {
+                            const uint64_t g = gaddr(2 * (threadIdx.x + T * i));
+                            const double x0 = out[i].x, y0 = out[i].y, x1 = out[i].z, y1 = out[i].w;
+                            acc += (x0 * x0 + y0 * y0) * zz_weight(g, n) + (x1 * x1 + y1 * y1) * zz_weight(g + 1, n);
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < NL; i++)
+                            *reinterpret_cast<float4 *>(slice + gaddr(2 * (threadIdx.x + T * i))) = out[i];
+                    }
                     __syncthreads();
                 }
-            }
-            for (uint64_t i = threadIdx.x; i < dim; i += T) {
-                const double x = slice[i].x, y = slice[i].y;
-                acc += (x * x + y * y) * zz_weight(i, n);
             }
         }
         __syncthreads();                      // the tile is read; reuse it as scratch
