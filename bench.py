@@ -87,13 +87,17 @@ def parse():
     p.add_argument("--haar", type=float, default=0.1)
     p.add_argument("--seed", type=int, default=2025)
     p.add_argument("--fuse-k", type=int, default=4, help="block size for greedy / passes (learned picks its own)")
-    p.add_argument("--scheduler", choices=["greedy", "passes", "learned"], default="learned",
+    p.add_argument("--scheduler", choices=["greedy", "passes", "learned", "window"], default="learned",
                    help="greedy: one block per sweep; passes: greedy multi-block sweeps; "
                         "learned: cost-model schedule (needs a trained model)")
     p.add_argument("--group", type=int, default=8, help="blocks per sweep for --scheduler passes")
+    p.add_argument("--pass-kernel", choices=["auto", "tile", "pipe"], default="auto",
+                   help="multi-block sweep kernel: one workgroup per tile or the pipelined one")
+    p.add_argument("--pass-tb", type=int, default=0,
+                   help="tile bits of multi-block sweeps (passes, window; 0 = minimal / 13)")
     p.add_argument("--costmodel", default=None, help="trained cost-model parameters (float32)")
     p.add_argument("--path", choices=["fused", "plain"], default="fused")
-    p.add_argument("--product", choices=["fp32", "split", "bf16x6"], default="bf16x6",
+    p.add_argument("--product", choices=["fp32", "split", "bf16x6", "fp16x3"], default="bf16x6",
                    help="matrix product of the fused blocks")
     p.add_argument("--no-plain", action="store_true", help="skip the FP64 and greedy comparison runs")
     p.add_argument("--no-cpu", action="store_true")
@@ -102,7 +106,8 @@ def parse():
     p.add_argument("--probe", action="store_true",
                    help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
     a = p.parse_args()
-    a.split = {"fp32": 0, "split": 1, "bf16x6": 2}[a.product]
+    a.split = {"fp32": 0, "split": 1, "bf16x6": 2, "fp16x3": 3}[a.product]
+    a.pk = {"auto": 0, "tile": 1, "pipe": 2}[a.pass_kernel]
     if a.n is None:
         a.n = {"large33": 33, "qrunes": 26, "stateio": 28}.get(a.workload, 30)
     if a.depth is None:
@@ -208,6 +213,7 @@ def load_costmodel(a):
 
 def fused_dtype(a):
     return {"bf16x6": "c64 (fp32 state; bf16x6 matrix-core product, fp32 accumulation; fp64 reductions)",
+            "fp16x3": "c64 (fp32 state; fp16x3 matrix-core product, fp32 accumulation; fp64 reductions)",
             "split": "c64 (fp32 mfma + bf16 correction; fp64 reductions)",
             "fp32": "c64 (fp32 mfma; fp64 reductions)"}[a.product]
 
@@ -230,11 +236,13 @@ def roofline_of(stats_list, a, headline, result):
         # peak in these flops is the BF16 dense peak / 6
         tf = d["flops"] / (d["ms"] * 1e9)
         intensity = d["flops"] / d["bytes"]
-        peak = BF16_PEAK_TFLOPS / 6 if a.product == "bf16x6" else FP32_PEAK_TFLOPS
+        # (FP16x3: three fp16 piece products, the FP16 dense peak / 3)
+        pieces = {"bf16x6": 6, "fp16x3": 3}.get(a.product)
+        peak = BF16_PEAK_TFLOPS / pieces if pieces else FP32_PEAK_TFLOPS
         result["mfma"] = {"tflops": tf, "peak": peak, "frac": tf / peak, "flop_per_byte": intensity,
                           "product": a.product}
-        if a.product == "bf16x6":
-            result["mfma"]["bf16_tflops_executed"] = 6 * tf
+        if pieces:
+            result["mfma"]["half_tflops_executed"] = pieces * tf
         if intensity * HBM_PEAK_GBS / 1e3 >= peak:                  # right of the ridge
             roof = {"bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
@@ -280,7 +288,7 @@ def main():
     headline = a.path
     steps = a.steps if a.steps is not None else (3 if headline == "fused" else 1)
     warmup = a.warmup if a.warmup is not None else (1 if headline == "fused" else 0)
-    sched = {"greedy": 0, "learned": 1, "passes": 2}[a.scheduler]
+    sched = {"greedy": 0, "learned": 1, "passes": 2, "window": 3}[a.scheduler]
     cm = load_costmodel(a) if sched == 1 else None
 
     plain = None
@@ -321,13 +329,15 @@ def main():
                                    "sweeps": int(greedy.launches), "fidelity_vs_fp64": sv_plain.fidelity(sv)}
         for _ in range(warmup):
             sv.reset()
-            sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group)
+            sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group, pass_tb=a.pass_tb,
+                   pass_kernel=a.pk)
         sv.sync()
         t0 = time.perf_counter()
         stats = []
         for _ in range(steps):
             sv.reset()
-            stats.append(sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group, profile=True))
+            stats.append(sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group,
+                                pass_tb=a.pass_tb, pass_kernel=a.pk, profile=True))
         sv.sync()
         wall = time.perf_counter() - t0
         head_stats = stats
@@ -335,6 +345,7 @@ def main():
             result["fidelity_vs_fp64"] = sv_plain.fidelity(sv)
         cl = stats[-1].classes()
         result["fused"] = {"k": int(stats[-1].fuse_k), "product": a.product, "scheduler": a.scheduler,
+                           "planner": "window" if stats[-1].planner else "block-first",
                            "sweeps": int(sum(cl.get(c, {}).get("launches", 0) for c in ("fused", "pass", "block64"))),
                            "plan_ms": stats[-1].plan_ms,
                            "blocks": int(stats[-1].blocks),

@@ -208,7 +208,10 @@ int qsv_apply_2q(qsv_state *s, int qa, int qb, const double *u);
  * (FP32 complex GEMM with FP32 accumulation; split = 1 adds the lo part of
  * the matrix, U = U_hi + U_lo; split = 2 runs the product as BF16x6 on the
  * BF16 matrix cores: matrix and state split into three bf16 pieces, the six
- * leading piece products accumulated in FP32).  No reference counterpart. */
+ * leading piece products accumulated in FP32; split = 3 runs it as FP16x3 on
+ * the FP16 matrix cores: matrix and state (scaled by a power of two derived
+ * from the state's norm) split into two fp16 pieces, the three leading piece
+ * products accumulated in FP32).  No reference counterpart. */
 int qsv_apply_block(qsv_state *s, const int *qubits, int k, const double *u, int split);
 
 /* ---- readout --------------------------------------------------------------- */
@@ -280,7 +283,7 @@ typedef struct qsv_run_opts {
     int exact;          /* 1: H / sqiSWAP use exact 1/sqrt(2)                   */
     int fuse_k;         /* 0: per-gate path; 2..5: fused blocks of <= k qubits  */
     int split;          /* fused blocks: 0 = FP32 product, 1 = FP32 + BF16 lo   */
-                        /* correction, 2 = BF16x6 product                       */
+                        /* correction, 2 = BF16x6 product, 3 = FP16x3 product   */
     int scheduler;      /* 0: greedy fusion, 1: learned cost model              */
     int profile;        /* 1: time every launch with events (per-class stats)   */
     int tile;           /* fused blocks: tile column bits 6, 7 or 8 (0 = 7);   */
@@ -288,10 +291,21 @@ typedef struct qsv_run_opts {
     int group;          /* scheduler 2: at most this many consecutive blocks    */
                         /* share one HBM sweep (0 or 1 = one block per sweep)   */
     int pass_tb;        /* scheduler 2: tile bits of such a pass (0 = minimal)  */
+    int pass_kernel;    /* multi-block sweeps: 0 = automatic (= 1), 1 = one     */
+                        /* workgroup per tile, 2 = pipelined (persistent, one   */
+                        /* workgroup per CU, HBM traffic of the neighbouring    */
+                        /* tiles overlapping the blocks; 64 KiB tiles of k = 4  */
+                        /* blocks with the BF16x6 / FP16x3 products)            */
 } qsv_run_opts;
 /* opts.scheduler: 0 = greedy blocks at fuse_k, one sweep each;
- *                 1 = learned (block size, grouping into sweeps, tile shapes);
- *                 2 = greedy blocks grouped greedily into multi-block sweeps. */
+ *                 1 = learned (planner, block size, grouping into sweeps,
+ *                     tile shapes);
+ *                 2 = greedy blocks grouped greedily into multi-block sweeps;
+ *                 3 = window planner: per sweep a window of pass_tb (0 = 13)
+ *                     tile bits holding qubits 0..3, every runnable gate
+ *                     inside it fused into blocks of <= fuse_k (4, 5)
+ *                     qubits, one sweep per window (states of >= 11 qubits;
+ *                     smaller ones fall back to scheduler 0). */
 
 /* Kernel classes for per-launch profiling. */
 enum {
@@ -324,6 +338,8 @@ typedef struct qsv_run_stats {
                             /* device time above starts after planning      */
     int64_t fuse_k;         /* block size the fused plan used (the learned  */
                             /* scheduler picks it), 0 on the per-gate path  */
+    int64_t planner;        /* fused plan: 0 block-first (plan_greedy),     */
+                            /* 1 window planner                             */
 } qsv_run_stats;
 
 /* Greedy fusion plan of a gate list (host only, no device needed): blocks of at
@@ -336,6 +352,15 @@ typedef struct qsv_run_stats {
 int qsv_fuse_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
                   int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
                   int64_t *ngates_out, int64_t *nblocks);
+
+/* Window plan of a gate list for an n-qubit state (host only, no device
+ * needed): the blocks (<= max_k = 4 or 5 qubits) of one sweep each lie in a
+ * window of tb qubits that holds qubits 0..min(3, n-1); outputs as
+ * qsv_fuse_plan plus window_out[b], the window of block b (a MEASURE/RESET
+ * record has a window of its own).  No reference counterpart. */
+int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
+                    int tb, int n, int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                    int64_t *ngates_out, int *window_out, int64_t *nblocks);
 
 int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngates,
                     const double *mats, const qsv_run_opts *opts,

@@ -250,9 +250,10 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
     QSV_HIP(hipEventRecord(e0, s->stream));
     int64_t blocks = 0;
     double pred_ms = 0, plan_ms = 0;
-    int k_used = 0;
+    int k_used = 0, planner = 0;
     if (o.fuse_k >= 2) {
-        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof, e0, &plan_ms, &k_used);
+        rc = run_fused(s, gates, ngates, mats, o, rng, outcomes, &blocks, &pred_ms, &prof, e0, &plan_ms, &k_used,
+                       &planner);
     } else {
         for (int64_t i = 0; i < ngates && !rc; i++) {
             rc = prof.begin();
@@ -284,6 +285,7 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
         stats->model_ms_pred = pred_ms;
         stats->plan_ms = plan_ms;
         stats->fuse_k = k_used;
+        stats->planner = planner;
         if (!rc) rc = prof.collect(stats);
     }
     if (!rc && prof.on() && o.fuse_k >= 2) {

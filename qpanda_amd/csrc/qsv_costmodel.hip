@@ -387,8 +387,33 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
             if (rc) return rc;
         }
     }
+    // window plans (qsv_window.cpp) at k = 4, 5, grouped by the same DP
+    int best_window = 0;
+    if (s->n >= 11) {
+        for (int k = 4; k <= 5; k++) {
+            std::vector<PlanItem> cand;
+            std::vector<int> win;
+            int rc = plan_window(gates, ng, mats, k, o.exact, kMaxPassTileBits, s->n, cand, win);
+            if (rc) return rc;
+            for (PlanItem &it : cand)
+                if (it.barrier < 0) it.qubits = widened_qubits(it.qubits, std::max(4, (int)it.qubits.size()));
+            std::vector<Launch> launches;
+            double total = 0;
+            rc = group_launches(s, cand, o, launches, &total);
+            if (rc) return rc;
+            if (total < best_total) {
+                best_total = total;
+                best_k = k;
+                best_window = 1;
+                best.clear();
+                rc = plan_window(gates, ng, mats, k, o.exact, kMaxPassTileBits, s->n, best, win);
+                if (rc) return rc;
+            }
+        }
+    }
     plan.swap(best);
     choice->fuse_k = best_k;
+    choice->window = best_window;
     choice->pred_ms = best_total;
     return QSV_OK;
 }

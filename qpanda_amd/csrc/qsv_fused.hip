@@ -40,6 +40,10 @@
 //          level.  Per K of 16 real rows it costs 6 v_mfma_f32_32x32x16_bf16
 //          (6 x 32 cycles) against 8 v_mfma_f32_32x32x2f32 (8 x 64 cycles),
 //          2.7x less matrix-core time, plus the VALU work of splitting B;
+//          split = 3 (FP16x3, qsv_mfma.h) splits matrix and scaled state into
+//          two fp16 pieces each and accumulates three piece products on the
+//          FP16 MFMA: 22 of FP32's 24 significand bits for half of BF16x6's
+//          matrix-core work;
 //   store: results go back to the planes, then 16-byte coalesced stores.
 // Per tile the kernel moves 2 * 8 * 2^(k+TC) bytes and performs
 // 2 * (2d)^2 * 2^TC flops in the FP32 product, so k = 4 is HBM-bound (8 flop/B,
@@ -50,8 +54,11 @@
 // (i/8)*4 + i%4.  32x32x16 bf16: A[i][k] in lane i + 32*(k/8), element k%8;
 // B[k][j] in lane j + 32*(k/8), element k%8; same D layout.
 #include "qsv_internal.h"
+#include "qsv_mfma.h"
 #include "qsv_sched.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace qsv {
@@ -64,6 +71,7 @@ struct FusedArgs {
     InsertSet tile_ins;      // L positions, to expand a tile index into its base
     int erow[32];            // compact tile position of row (amplitude) r
     int col_rank[8];         // rank in L of the i-th free tile (column) bit
+    float bscale, binv;      // FP16x3: state scale 2^s and 2^-s
 };
 
 // acc[h] += M B over one K-block pair of bf16 pieces.  ap[p][h][sb]: piece p
@@ -100,7 +108,7 @@ __device__ __forceinline__ void bf16x6_product(floatx16 (&acc)[HALVES], const bf
 }
 
 // KQ: block qubits (4, 5).  TC: column bits per tile (6, 7, 8 -> 2, 4, 8 waves).
-// MODE: 0 FP32 product, 1 FP32 + BF16 lo correction, 2 BF16x6.
+// MODE: 0 FP32 product, 1 FP32 + BF16 lo correction, 2 BF16x6, 3 FP16x3.
 template <int KQ, int TC, int MODE>
 __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, uint64_t ntiles, FusedArgs fa,
                                                          const float *__restrict__ mops)
@@ -115,6 +123,8 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     constexpr int LOBLK = D / 8;             // bf16 K-blocks (16 real rows = 8 amplitudes)
     constexpr bool SPLIT = MODE == 1;
     constexpr bool X6 = MODE == 2;
+    constexpr bool H3 = MODE == 3;
+    constexpr bool NOHI = X6 || H3;           // no FP32 A operand
     __shared__ float planes[2][TSIZE];     // [0] = re, [1] = im
     float *const pre = planes[0];
     float *const pim = planes[1];
@@ -135,9 +145,10 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     }
 
     // A operands (the matrix), operand-ready layout: mops[(h*KSTEPS + s)*64 + lane]
-    float ahi[X6 ? 1 : HALVES][X6 ? 1 : KSTEPS];
+    float ahi[NOHI ? 1 : HALVES][NOHI ? 1 : KSTEPS];
     bf16x8 alo[SPLIT ? HALVES : 1][SPLIT ? LOBLK : 1];
-    bf16x8 ap[3][HALVES][LOBLK];
+    bf16x8 ap[X6 ? 3 : 1][X6 ? HALVES : 1][X6 ? LOBLK : 1];
+    f16x8 ah[H3 ? 2 : 1][H3 ? HALVES : 1][H3 ? LOBLK : 1];
     if (X6) {
         const bf16x8 *p = reinterpret_cast<const bf16x8 *>(mops);
 #pragma unroll
@@ -145,12 +156,22 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
 #pragma unroll
             for (int h = 0; h < HALVES; h++)
 #pragma unroll
-                for (int sb = 0; sb < LOBLK; sb++) ap[q][h][sb] = p[((q * HALVES + h) * LOBLK + sb) * 64 + lane];
+                for (int sb = 0; sb < LOBLK; sb++)
+                    ap[X6 ? q : 0][X6 ? h : 0][X6 ? sb : 0] = p[((q * HALVES + h) * LOBLK + sb) * 64 + lane];
+    } else if (H3) {
+        const f16x8 *p = reinterpret_cast<const f16x8 *>(mops);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int h = 0; h < HALVES; h++)
+#pragma unroll
+                for (int sb = 0; sb < LOBLK; sb++)
+                    ah[H3 ? q : 0][H3 ? h : 0][H3 ? sb : 0] = p[((q * HALVES + h) * LOBLK + sb) * 64 + lane];
     } else {
 #pragma unroll
         for (int h = 0; h < HALVES; h++)
 #pragma unroll
-            for (int s = 0; s < KSTEPS; s++) ahi[X6 ? 0 : h][X6 ? 0 : s] = mops[(h * KSTEPS + s) * 64 + lane];
+            for (int s = 0; s < KSTEPS; s++) ahi[NOHI ? 0 : h][NOHI ? 0 : s] = mops[(h * KSTEPS + s) * 64 + lane];
     }
     if (SPLIT) {
         const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(mops + HALVES * KSTEPS * 64);
@@ -160,11 +181,14 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
             for (int sb = 0; sb < LOBLK; sb++) alo[SPLIT ? h : 0][SPLIT ? sb : 0] = lo[(h * LOBLK + sb) * 64 + lane];
     }
 
+    // FP16x3: the tile sits in LDS scaled by 2^s (exact), so the split needs
+    // no multiply; the store scales back
+    const float ls = H3 ? fa.bscale : 1.0f, li = H3 ? fa.binv : 1.0f;
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
-        pre[e] = buf[i].x; pim[e] = buf[i].y;
-        pre[e + 1] = buf[i].z; pim[e + 1] = buf[i].w;
+        pre[e] = buf[i].x * ls; pim[e] = buf[i].y * ls;
+        pre[e + 1] = buf[i].z * ls; pim[e + 1] = buf[i].w * ls;
     }
     __syncthreads();
 
@@ -184,14 +208,16 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     float bv[KSTEPS];
 #pragma unroll
     for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[fa.erow[s] | ecol];
-    if (X6) {
+    if constexpr (X6) {
         bf16x6_product<HALVES, LOBLK>(acc, ap, bv);
+    } else if constexpr (H3) {
+        fp16x3_product<HALVES, LOBLK>(acc, ah, bv, 1.0f);
     } else {
 #pragma unroll
         for (int s = 0; s < KSTEPS; s++)
 #pragma unroll
             for (int h = 0; h < HALVES; h++)
-                acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[X6 ? 0 : h][X6 ? 0 : s], bv[s], acc[h], 0, 0, 0);
+                acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[NOHI ? 0 : h][NOHI ? 0 : s], bv[s], acc[h], 0, 0, 0);
     }
     if (SPLIT) {
 #pragma unroll
@@ -220,8 +246,8 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
         float4 v;
-        v.x = pre[e]; v.y = pim[e];
-        v.z = pre[e + 1]; v.w = pim[e + 1];
+        v.x = pre[e] * li; v.y = pim[e] * li;
+        v.z = pre[e + 1] * li; v.w = pim[e + 1] * li;
         *reinterpret_cast<float4 *>(a + off[i]) = v;
     }
 }
@@ -235,6 +261,9 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
 // a0), a2 = bf16(M - a0 - a1), residuals in FP64) in the permuted K order,
 // value a_p[32h + l%32][2(8sb + e) + l/32] at (((p*HALVES + h)*(D/8) + sb)*64
 // + l)*8 + e: 1.5x the floats of M_hi, the size of the split = 1 layout.
+// split = 3: fp16 pieces p = 0, 1 (a0 = fp16(M), a1 = fp16(M - a0), each
+// rounded once from FP64) in the same permuted K order and index formula
+// with 2 pieces: the floats of M_hi.
 static uint16_t to_bf16(float f)
 {
     uint32_t x;
@@ -261,6 +290,22 @@ void fused_operands(int kq, const double *u, int split, float *out)
         if ((i & 1) == 0) return (j & 1) == 0 ? ur : -ui;
         return (j & 1) == 0 ? ui : ur;
     };
+    if (split == 3) {
+        _Float16 *p = reinterpret_cast<_Float16 *>(out);
+        const int lob = d / 8;
+        for (int h = 0; h < halves; h++)
+            for (int sb = 0; sb < lob; sb++)
+                for (int l = 0; l < 64; l++)
+                    for (int e = 0; e < 8; e++) {
+                        double r = M(32 * h + (l & 31), 2 * (8 * sb + e) + (l >> 5));
+                        for (int q = 0; q < 2; q++) {
+                            const _Float16 b = (_Float16)r;
+                            p[(((q * halves + h) * lob + sb) * 64 + l) * 8 + e] = b;
+                            r -= (double)b;
+                        }
+                    }
+        return;
+    }
     if (split == 2) {
         uint16_t *p = reinterpret_cast<uint16_t *>(out);
         const int lob = d / 8;
@@ -298,7 +343,8 @@ size_t fused_operand_floats(int kq, int split)
     const int d = 1 << kq, halves = (2 * d) / 32;
     const size_t hi = (size_t)halves * d * 64;
     // split 1: the bf16 lo part is half the bytes of the hi part; split 2:
-    // three bf16 pieces, the same 1.5x
+    // three bf16 pieces, the same 1.5x; split 3: two fp16 pieces, 1x
+    if (split == 3) return hi;
     return hi + (split ? hi / 2 : 0);
 }
 
@@ -307,7 +353,8 @@ static void launch_kq_tc(qsv_state *s, uint64_t ntiles, const FusedArgs &fa, con
 {
     float2 *a = (float2 *)s->amps;
     const dim3 g((unsigned)ntiles), t(2 << TC);
-    if (split == 2) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 2>), g, t, 0, s->stream, a, ntiles, fa, mops);
+    if (split == 3) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 3>), g, t, 0, s->stream, a, ntiles, fa, mops);
+    else if (split == 2) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 2>), g, t, 0, s->stream, a, ntiles, fa, mops);
     else if (split) hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 1>), g, t, 0, s->stream, a, ntiles, fa, mops);
     else hipLaunchKernelGGL((k_fused_mfma<KQ, TC, 0>), g, t, 0, s->stream, a, ntiles, fa, mops);
 }
@@ -320,9 +367,11 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused MFMA blocks need a c64 state");
     if (kq != 4 && kq != 5) return fail(QSV_ERR_ARG, "fused block size must be 4 or 5");
     if (tc < 6 || tc > 8) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
-    if (split < 0 || split > 2) return fail(QSV_ERR_ARG, "split must be 0, 1 or 2");
+    if (split < 0 || split > 3) return fail(QSV_ERR_ARG, "split must be 0, 1, 2 or 3");
     if (s->n < kq + tc) return fail(QSV_ERR_ARG, "state too small for a fused tile");
     FusedArgs fa{};
+    fa.bscale = s->f16_scale;
+    fa.binv = 1.0f / s->f16_scale;
     uint64_t qmask = 0;
     for (int j = 0; j < kq; j++) qmask |= uint64_t(1) << qubits[j];
     // the tc lowest free bits are the tile's columns
@@ -369,6 +418,20 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
     return QSV_OK;
 }
 
+// FP16x3 state scale from the state's FP64 norm: s = 2^(15 - e) with 2^e >=
+// the largest possible amplitude sqrt(norm), so every |x s| stays below 2^15.
+int f16_state_scale(qsv_state *s)
+{
+    double nrm = 0;
+    int rc = reduce_norm2(s, &nrm);
+    if (rc) return rc;
+    int e = 0;
+    if (nrm > 0 && std::isfinite(nrm)) e = (int)std::ceil(0.5 * std::log2(nrm));
+    e = std::max(-60, std::min(60, e));
+    s->f16_scale = std::ldexp(1.0f, 15 - e);
+    return QSV_OK;
+}
+
 // qsv_apply_block on a c64 state: blocks of 4 or 5 qubits go to the MFMA
 // kernel; smaller blocks are widened with identity on the lowest free qubits
 // (same sweep, same kernel); states too small for a tile use the FP64 block
@@ -396,6 +459,10 @@ int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, i
                     w[2 * (R * D + Cc) + 1] = u[2 * (r * d + c) + 1];
                 }
         uu = w.data();
+    }
+    if (split == 3) {
+        int rc = f16_state_scale(s);
+        if (rc) return rc;
     }
     std::vector<float> ops(fused_operand_floats(kq, split));
     fused_operands(kq, uu, split, ops.data());

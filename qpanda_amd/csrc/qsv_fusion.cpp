@@ -325,16 +325,25 @@ void widen_to(PlanItem &it, int kq)
 
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &o, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms,
-              Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used)
+              Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used, int *planner)
 {
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "the fused path runs on c64 states");
     const auto t_plan = std::chrono::steady_clock::now();
     if (o.tile != 0 && (o.tile < 6 || o.tile > 8)) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
     std::vector<PlanItem> plan;
+    std::vector<int> window;
     int rc;
     SchedChoice choice;
+    const bool windowed = o.scheduler == 3 && s->n >= 11;
+    if (o.pass_tb != 0 && (o.pass_tb < 11 || o.pass_tb > kMaxPassTileBits))
+        return fail(QSV_ERR_ARG, "pass tile bits must be 0 or 11..13");
     if (o.scheduler == 1) {
         rc = plan_learned(s, gates, ngates, mats, o, plan, &choice);
+    } else if (windowed) {
+        choice.fuse_k = std::max(4, std::min(o.fuse_k, 5));
+        choice.window = 1;
+        rc = plan_window(gates, ngates, mats, choice.fuse_k, o.exact, o.pass_tb ? o.pass_tb : kMaxPassTileBits,
+                         s->n, plan, window);
     } else {
         rc = plan_greedy(gates, ngates, mats, std::min(o.fuse_k, 5), o.exact, plan);
         choice.fuse_k = o.fuse_k;
@@ -355,6 +364,30 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
     double pred = 0;
     if (o.scheduler == 1 && s->cm) {
         rc = group_launches(s, plan, o, launches, &pred);
+    } else if (windowed) {
+        // one sweep per window (several beyond the pass kernel's block limit)
+        const int tb_min = o.pass_tb ? o.pass_tb : 11;
+        for (size_t i = 0; i < plan.size();) {
+            size_t j = i + 1;
+            while (j < plan.size() && plan[i].barrier < 0 && plan[j].barrier < 0 && window[j] == window[i]) j++;
+            for (size_t b = i; b < j; b += kMaxPassBlocks) {
+                Launch L;
+                L.first = b;
+                L.count = (int)std::min<size_t>(kMaxPassBlocks, j - b);
+                if (plan[b].barrier >= 0) {
+                    L.count = 1;
+                } else if (L.count == 1) {
+                    const int kq = (int)plan[b].qubits.size();
+                    L.tc = mfma[b] ? std::max(6, std::min(kDefaultTileCols, s->n - kq)) : 0;
+                } else {
+                    std::vector<const std::vector<int> *> q;
+                    for (int g = 0; g < L.count; g++) q.push_back(&plan[b + g].qubits);
+                    L.tb = std::max(pass_tile_bits(q, s->n), std::min(tb_min, s->n));
+                }
+                launches.push_back(L);
+            }
+            i = j;
+        }
     } else {
         qsv_run_opts g = o;
         if (o.scheduler != 2) g.group = 1;
@@ -382,10 +415,15 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
             return hip_fail(e, "operand upload");
         }
     }
+    if (o.split == 3 && (rc = f16_state_scale(s))) {
+        if (dops) (void)hipFree(dops);
+        return rc;
+    }
     const double N = (double)s->dim;
     int64_t nblocks = 0;
     *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
     *k_used = choice.fuse_k;
+    *planner = choice.window;
     QSV_HIP(hipEventRecord(start, s->stream));       // device time starts after planning
     rc = QSV_OK;
     if (prof->on()) {
@@ -438,7 +476,7 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
             tb = (int)it0.qubits.size() + tc;
             rc = launch_fused_dev(s, it0.qubits.data(), (int)it0.qubits.size(), mp[0], o.split, tc);
         } else {
-            rc = launch_pass_dev(s, qsets, mp, o.split, tb);
+            rc = launch_pass_dev(s, qsets, mp, o.split, tb, o.pass_kernel);
             cls = QSV_K_PASS;
         }
         int64_t tag = -1;

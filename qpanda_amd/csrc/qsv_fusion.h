@@ -37,6 +37,6 @@ private:
 // size of the plan.
 int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
               const qsv_run_opts &opts, int32_t *rng, int *outcomes, int64_t *blocks,
-              double *pred_ms, Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used);
+              double *pred_ms, Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used, int *planner);
 
 }  // namespace qsv

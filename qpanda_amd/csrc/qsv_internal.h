@@ -72,6 +72,7 @@ struct qsv_state {
     int device = 0;
     int64_t launches = 0;
     int64_t tie_resolutions = 0;
+    float f16_scale = 32768.0f;      // FP16x3 products: state scale (f16_state_scale)
     struct qsv_costmodel *cm = nullptr;   // attached cost model (not owned)
     std::vector<float> tlog_feats;        // per-block features of the last profiled fused run
     std::vector<float> tlog_ms;           // and their measured device ms
@@ -111,6 +112,8 @@ int marginals(qsv_state *s, const int *qubits, int m, double *out);
 
 // Fused FP32 MFMA block (qsv_fused.hip).
 int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int split);
+// Sets s->f16_scale from the state's norm (one FP64 reduction).
+int f16_state_scale(qsv_state *s);
 
 // Build an insert set from a bit mask (ascending positions).
 inline InsertSet insert_set_from_mask(uint64_t mask)

@@ -1,6 +1,6 @@
 // qsv_pass.hip -- several consecutive fused blocks in one HBM sweep.
 //
-// A pass takes G <= 8 consecutive blocks of a fusion plan.  L = qubits 0..3 u
+// A pass takes G <= 16 consecutive blocks of a fusion plan.  L = qubits 0..3 u
 // (the blocks' qubits) u {lowest other qubits until |L| = TB}, TB = 11..13
 // (2^TB complex64 amplitudes = 16..64 KiB of LDS, so several workgroups share
 // a CU); a tile is the 2^TB amplitudes whose non-L bits are fixed, made of
@@ -32,12 +32,18 @@
 // five resident 32 KiB tiles per CU measured 4% slower as well.)  Whether a grouping pays is a
 // scheduling decision (greedy grouping, or the cost model in qsv_costmodel.hip).
 #include "qsv_internal.h"
+#include "qsv_mfma.h"
 #include "qsv_sched.h"
 
 namespace qsv {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Threads per workgroup: 512 (8 waves) for 64 KiB tiles of k = 4 blocks, so
+// that the two workgroups LDS admits per CU still give each SIMD 4 waves (123
+// VGPRs); else 256 (the k = 5 product needs up to 210 VGPRs).
+__host__ __device__ constexpr int pass_threads(int tb, bool any5) { return tb == 13 && !any5 ? 512 : 256; }
 
 // Bank bits 1..4 XOR-ed into the LDS float index by bits 5, 6, 7, 8.
 constexpr int kLdsSwizzle[4] = {0x1E, 0x12, 0x14, 0x1C};
@@ -64,6 +70,7 @@ struct PassArgs {
     int lpos[16];           // compact tile bit j -> qubit
     uint64_t hioff[16];     // pdep(512 i, L): address part of load i (i < 2^(TB-9))
     int nblocks;
+    float bscale, binv;     // FP16x3: state scale 2^s and 2^-s
     PassBlock b[kMaxPassBlocks];
 };
 
@@ -114,16 +121,28 @@ __device__ __forceinline__ void pass_bf16x6(floatx16 (&acc)[HALVES], const bf16x
     }
 }
 
-// MODE: 0 FP32 product, 1 FP32 + BF16 lo correction, 2 BF16x6.
-template <int KQ, int MODE>
-__device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlock &pb, int lane, int wave)
+// MODE: 0 FP32 product, 1 FP32 + BF16 lo correction, 2 BF16x6, 3 FP16x3.
+// NW: waves per workgroup.
+template <int KQ, int MODE, int NW>
+__device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlock &pb, int lane, int wave,
+                                           float bscale, float binv)
 {
     constexpr int D = 1 << KQ, HALVES = (2 * D) / 32, KSTEPS = D, LOBLK = D / 8;
-    constexpr bool SPLIT = MODE == 1, X6 = MODE == 2;
-    float ahi[X6 ? 1 : HALVES][X6 ? 1 : KSTEPS];
+    constexpr bool SPLIT = MODE == 1, X6 = MODE == 2, H3 = MODE == 3, NOHI = X6 || H3;
+    float ahi[NOHI ? 1 : HALVES][NOHI ? 1 : KSTEPS];
     bf16x8 alo[SPLIT ? HALVES : 1][SPLIT ? LOBLK : 1];
     bf16x8 ap[X6 ? 3 : 1][X6 ? HALVES : 1][X6 ? LOBLK : 1];
-    if (X6) {
+    f16x8 ah[H3 ? 2 : 1][H3 ? HALVES : 1][H3 ? LOBLK : 1];
+    if (H3) {
+        const f16x8 *p = reinterpret_cast<const f16x8 *>(pb.mops);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int h = 0; h < HALVES; h++)
+#pragma unroll
+                for (int sb = 0; sb < LOBLK; sb++)
+                    ah[H3 ? q : 0][H3 ? h : 0][H3 ? sb : 0] = p[((q * HALVES + h) * LOBLK + sb) * 64 + lane];
+    } else if (X6) {
         const bf16x8 *p = reinterpret_cast<const bf16x8 *>(pb.mops);
 #pragma unroll
         for (int q = 0; q < 3; q++)
@@ -136,7 +155,7 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
 #pragma unroll
         for (int h = 0; h < HALVES; h++)
 #pragma unroll
-            for (int s = 0; s < KSTEPS; s++) ahi[X6 ? 0 : h][X6 ? 0 : s] = pb.mops[(h * KSTEPS + s) * 64 + lane];
+            for (int s = 0; s < KSTEPS; s++) ahi[NOHI ? 0 : h][NOHI ? 0 : s] = pb.mops[(h * KSTEPS + s) * 64 + lane];
     }
     if (SPLIT) {
         const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(pb.mops + HALVES * KSTEPS * 64);
@@ -147,7 +166,7 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
     }
     const int chunks = (1 << pb.ncolbits) >> 5;
     const float *const bplane = (lane >> 5) ? pim : pre;
-    for (int chunk = wave; chunk < chunks; chunk += 4) {
+    for (int chunk = wave; chunk < chunks; chunk += NW) {
         const int col = chunk * 32 + (lane & 31);
         int ecol = 0;
         for (int i = 0; i < pb.ncolbits; i++)
@@ -162,12 +181,16 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
         for (int s = 0; s < KSTEPS; s++) bv[s] = bplane[pb.erow[s] ^ ecol];
         if constexpr (X6) {
             pass_bf16x6<HALVES, LOBLK>(acc, ap, bv);
+        } else if constexpr (H3) {
+            fp16x3_product<HALVES, LOBLK>(acc, ah, bv, bscale);
+#pragma unroll
+            for (int h = 0; h < HALVES; h++) acc[h] *= binv;
         } else {
 #pragma unroll
             for (int s = 0; s < KSTEPS; s++)
 #pragma unroll
                 for (int h = 0; h < HALVES; h++)
-                    acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[X6 ? 0 : h][X6 ? 0 : s], bv[s], acc[h], 0, 0, 0);
+                    acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ahi[NOHI ? 0 : h][NOHI ? 0 : s], bv[s], acc[h], 0, 0, 0);
         }
         if (SPLIT) {
             // M_lo in the permuted K order of the operand layout: the B values
@@ -198,14 +221,23 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
 // the split, A of the BF16 correction).
 template <int MODE>
 struct Ops4 {
-    float hi[MODE == 2 ? 1 : 16];
+    float hi[MODE >= 2 ? 1 : 16];
     bf16x8 lo[MODE == 1 ? 2 : 1];
     bf16x8 p[MODE == 2 ? 3 : 1][1][MODE == 2 ? 2 : 1];   // BF16x6 pieces [piece][half][K-block]
+    f16x8 h[MODE == 3 ? 2 : 1][1][MODE == 3 ? 2 : 1];    // FP16x3 pieces [piece][half][K-block]
 };
 
 template <int MODE>
 __device__ __forceinline__ void load_ops4(Ops4<MODE> &o, const float *mops, int lane)
 {
+    if (MODE == 3) {
+        const f16x8 *p = reinterpret_cast<const f16x8 *>(mops);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int sb = 0; sb < 2; sb++) o.h[MODE == 3 ? q : 0][0][MODE == 3 ? sb : 0] = p[(q * 2 + sb) * 64 + lane];
+        return;
+    }
     if (MODE == 2) {
         const bf16x8 *p = reinterpret_cast<const bf16x8 *>(mops);
 #pragma unroll
@@ -215,7 +247,7 @@ __device__ __forceinline__ void load_ops4(Ops4<MODE> &o, const float *mops, int 
         return;
     }
 #pragma unroll
-    for (int s = 0; s < 16; s++) o.hi[MODE == 2 ? 0 : s] = mops[s * 64 + lane];
+    for (int s = 0; s < 16; s++) o.hi[MODE >= 2 ? 0 : s] = mops[s * 64 + lane];
     if (MODE == 1) {
         const bf16x8 *lo = reinterpret_cast<const bf16x8 *>(mops + 16 * 64);
 #pragma unroll
@@ -230,13 +262,14 @@ __device__ __forceinline__ void load_ops4(Ops4<MODE> &o, const float *mops, int 
 // computed once per block), bits 5.. from the wave-uniform chunk index (scalar
 // instructions); the 16 row addresses of a chunk serve the B reads and the
 // write-back (a lane writes rows 4j' + 2 hf + {0, 1} of both planes).
-template <int TB, int MODE>
+template <int TB, int MODE, int NW>
 __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, const Ops4<MODE> &o, int lane,
-                                            int wave)
+                                            int wave, float bscale, float binv)
 {
     constexpr bool SPLIT = MODE == 1;
     constexpr int NCB = TB - 4;                      // column bits
-    constexpr int PER = ((1 << NCB) >> 5) / 4;       // chunks per wave: 1, 2, 4
+    constexpr int PER = ((1 << NCB) >> 5) / NW;      // chunks per wave: 1, 2, 4
+    static_assert(PER >= 1, "fewer 32-column chunks than waves");
     constexpr int PLANE = 1 << TB;                   // im plane offset (floats)
     const int hf = lane >> 5;
     // byte offsets: the swizzled index is linear, so an element's offset is
@@ -256,7 +289,7 @@ __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, con
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     char *const base = reinterpret_cast<char *>(pre);
     for (int c = 0; c < PER; c++) {
-        const int chunk = wv + 4 * c;                // wave-uniform
+        const int chunk = wv + NW * c;               // wave-uniform
         int ehi = 0;
 #pragma unroll
         for (int i = 5; i < NCB; i++)
@@ -272,10 +305,14 @@ __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, con
             floatx16 a1[1] = {acc};
             pass_bf16x6<1, 2>(a1, o.p, bv);
             acc = a1[0];
+        } else if constexpr (MODE == 3) {
+            floatx16 a1[1] = {acc};
+            fp16x3_product<1, 2>(a1, o.h, bv, bscale);
+            acc = a1[0] * binv;
         } else {
 #pragma unroll
             for (int s = 0; s < 16; s++)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(o.hi[MODE == 2 ? 0 : s], bv[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(o.hi[MODE >= 2 ? 0 : s], bv[s], acc, 0, 0, 0);
         }
         if (SPLIT) {
 #pragma unroll
@@ -304,7 +341,8 @@ template <int TB, int MODE, bool ANY5>
 __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t ntiles, const PassArgs &pa)
 {
     constexpr int TSIZE = 1 << TB;
-    constexpr int NLD = TSIZE / 512;         // 16-byte loads per thread
+    constexpr int NT = pass_threads(TB, ANY5), NW = NT / 64, TBITS = NT == 512 ? 9 : 8;
+    constexpr int NLD = TSIZE / (2 * NT);    // 16-byte loads per thread
     __shared__ float planes[2][TSIZE];
     float *const pre = planes[0];
     float *const pim = planes[1];
@@ -312,53 +350,188 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     const uint64_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     const uint64_t base = expand_index(tile, pa.tile_ins);
-    // element 2 tid + 512 i sits at base + pdep(2 tid, L) + pdep(512 i, L);
+    // element 2 tid + 2 NT i sits at base + pdep(2 tid, L) + pdep(2 NT i, L);
     // bit 0 of L is qubit 0, so element + 1 is the next amplitude in memory
     uint64_t toff = 0;
 #pragma unroll
-    for (int j = 1; j < 9; j++)
+    for (int j = 1; j <= TBITS; j++)
         if ((tid >> (j - 1)) & 1) toff |= uint64_t(1) << pa.lpos[j];
     float4 buf[NLD];
 #pragma unroll
     for (int i = 0; i < NLD; i++) buf[i] = *reinterpret_cast<const float4 *>(a + base + toff + pa.hioff[i]);
-    // elements 2 tid + 512 i: bits 5..8 come from tid alone, so the swizzle is
+    // elements 2 tid + 2 NT i: bits 5..8 come from tid alone, so the swizzle is
     // one XOR per thread; it keeps bit 0, so element + 1 follows at index + 1
     const int sw = lds_swizzle(2 * tid) ^ (2 * tid);
+    // FP16x3: the tile sits in LDS scaled by 2^s (exact) from staging to the
+    // store, so the blocks split it without a multiply on either side
+    const float ls = MODE == 3 ? pa.bscale : 1.0f, li = MODE == 3 ? pa.binv : 1.0f;
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
-        const int e = (2 * (tid + 256 * i)) ^ sw;
-        pre[e] = buf[i].x; pim[e] = buf[i].y;
-        pre[e + 1] = buf[i].z; pim[e + 1] = buf[i].w;
+        const int e = (2 * (tid + NT * i)) ^ sw;
+        pre[e] = buf[i].x * ls; pim[e] = buf[i].y * ls;
+        pre[e + 1] = buf[i].z * ls; pim[e + 1] = buf[i].w * ls;
     }
     __syncthreads();
     if (ANY5) {
         for (int g = 0; g < pa.nblocks; g++) {
-            if (pa.b[g].kq == 4) pass_apply<4, MODE>(pre, pim, pa.b[g], lane, wave);
-            else pass_apply<5, MODE>(pre, pim, pa.b[g], lane, wave);
+            if (pa.b[g].kq == 4) pass_apply<4, MODE, NW>(pre, pim, pa.b[g], lane, wave, 1.0f, 1.0f);
+            else pass_apply<5, MODE, NW>(pre, pim, pa.b[g], lane, wave, 1.0f, 1.0f);
             __syncthreads();
         }
     } else {
         for (int g = 0; g < pa.nblocks; g++) {
             Ops4<MODE> ops;
             load_ops4<MODE>(ops, pa.b[g].mops, lane);
-            pass_apply4<TB, MODE>(pre, pa.b[g], ops, lane, wave);
+            pass_apply4<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, 1.0f, 1.0f);
             __syncthreads();
         }
     }
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
-        const int e = (2 * (tid + 256 * i)) ^ sw;
+        const int e = (2 * (tid + NT * i)) ^ sw;
         float4 v;
-        v.x = pre[e]; v.y = pim[e];
-        v.z = pre[e + 1]; v.w = pim[e + 1];
+        v.x = pre[e] * li; v.y = pim[e] * li;
+        v.z = pre[e + 1] * li; v.w = pim[e + 1] * li;
         *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
     }
 }
 
+static_assert(sizeof(PassArgs) <= 4096, "pass arguments must fit the 4 KiB kernel-argument budget");
+
 template <int TB, int MODE, bool ANY5>
-__global__ __launch_bounds__(256) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
+__global__ __launch_bounds__(pass_threads(TB, ANY5)) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
 {
     fused_pass_body<TB, MODE, ANY5>(a, ntiles, pa);
+}
+
+// Pipelined sweep over 64 KiB tiles of k = 4 blocks.  One persistent
+// workgroup per CU owns two LDS tile buffers and two kinds of waves:
+//   compute waves (8): apply the pass's blocks to tile i in buffer i % 2,
+//                      a barrier after each block (as in k_fused_pass);
+//   memory waves  (8): meanwhile drain tile i - 1 from the other buffer to
+//                      HBM, stage tile i + 1 (loaded one iteration earlier)
+//                      into it and issue the loads of tile i + 2.
+// Every wave passes the same G barriers per tile (G = pa.nblocks, uniform),
+// the roles just run different code between them.  Splitting the roles keeps
+// the memory waves' outstanding loads out of the compute waves' vmcnt, so the
+// operand loads of a block never wait for the next tile's HBM loads.  A tile
+// would take max(G blocks of compute, one HBM read + write) instead of their
+// sum, but the blocks are bound by VALU and LDS issue, and with 2 compute
+// waves per SIMD (against 4 in k_fused_pass with 512 threads) they run ~2x
+// slower: at 30 qubits, windows of ~5.8 k = 4 blocks, 6.1 ms per sweep
+// against 4.9 ms for k_fused_pass, so it is not the automatic choice.
+constexpr int kPipeCompute = 8, kPipeMem = 8;     // waves per role
+constexpr int kPipeThreads = 64 * (kPipeCompute + kPipeMem);
+constexpr int kPipeMemThreads = 64 * kPipeMem;
+
+template <int MODE>
+__global__ __launch_bounds__(kPipeThreads) void k_fused_pass_pipe(float2 *__restrict__ a, uint64_t ntiles,
+                                                                  PassArgs pa)
+{
+    constexpr int TB = 13, TSIZE = 1 << TB;
+    constexpr int NLD = TSIZE / (2 * kPipeMemThreads);  // 16-byte loads per memory thread
+    __shared__ float planes[2][2][TSIZE];             // [buffer][re, im][element]
+    // the role branch must be wave-uniform (a scalar branch): a wave must never
+    // step through the other role's barriers with its lanes masked off
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint64_t first = blockIdx.x;
+    if (first >= ntiles) return;                      // whole workgroup
+    const uint64_t T = (ntiles - 1 - first) / gridDim.x + 1;   // tiles of this workgroup
+    const int G = pa.nblocks;
+    if (wave >= kPipeCompute) {
+        const int mt = tid - 64 * kPipeCompute;       // 0 .. 511
+        // element 2 mt + 1024 j of a tile sits at base + pdep(2 mt, L) + pdep(1024 j, L)
+        static_assert(kPipeMemThreads == 512, "memory-thread index bits");
+        uint64_t toff = 0;
+#pragma unroll
+        for (int j = 1; j <= 9; j++)
+            if ((mt >> (j - 1)) & 1) toff |= uint64_t(1) << pa.lpos[j];
+        const int sw = lds_swizzle(2 * mt) ^ (2 * mt);
+        const float ls = MODE == 3 ? pa.bscale : 1.0f, li = MODE == 3 ? pa.binv : 1.0f;   // as k_fused_pass
+        // two register sets: the loads of tile i + 2 are issued while tile i
+        // is computed and staged one iteration later, so they have a whole
+        // tile period to arrive
+        float4 bufA[NLD], bufB[NLD];
+        auto base_of = [&](uint64_t i) { return expand_index(first + i * gridDim.x, pa.tile_ins) + toff; };
+        auto load = [&](float4 (&buf)[NLD], uint64_t i) {
+            const uint64_t b = base_of(i);
+#pragma unroll
+            for (int j = 0; j < NLD; j++) buf[j] = *reinterpret_cast<const float4 *>(a + b + pa.hioff[j]);
+        };
+        auto stage = [&](const float4 (&buf)[NLD], int bi) {
+            float *pre = planes[bi][0], *pim = planes[bi][1];
+#pragma unroll
+            for (int j = 0; j < NLD; j++) {
+                const int e = (2 * (mt + kPipeMemThreads * j)) ^ sw;
+                pre[e] = buf[j].x * ls; pim[e] = buf[j].y * ls;
+                pre[e + 1] = buf[j].z * ls; pim[e + 1] = buf[j].w * ls;
+            }
+        };
+        auto drain = [&](int bi, uint64_t i) {
+            const float *pre = planes[bi][0], *pim = planes[bi][1];
+            const uint64_t b = base_of(i);
+#pragma unroll
+            for (int j = 0; j < NLD; j++) {
+                const int e = (2 * (mt + kPipeMemThreads * j)) ^ sw;
+                float4 v;
+                v.x = pre[e] * li; v.y = pim[e] * li;
+                v.z = pre[e + 1] * li; v.w = pim[e + 1] * li;
+                *reinterpret_cast<float4 *>(a + b + pa.hioff[j]) = v;
+            }
+        };
+        load(bufA, 0);
+        if (T > 1) load(bufB, 1);
+        stage(bufA, 0);
+        __syncthreads();
+        // iterations in pairs so that each register set has a fixed role:
+        // even i computes in buffer 0, drains tile i - 1 from buffer 1, stages
+        // tile i + 1 (bufB) into it and loads tile i + 2 into bufA; odd i the
+        // mirror image.  A thread drains and restages only its own elements.
+        for (uint64_t i = 0; i < T; i += 2) {
+            for (int g = 0; g < G; g++) {
+                if (g == 0) {
+                    if (i > 0) drain(1, i - 1);
+                    if (i + 1 < T) stage(bufB, 1);
+                    if (i + 2 < T) load(bufA, i + 2);
+                }
+                __syncthreads();
+            }
+            if (i + 1 >= T) break;
+            for (int g = 0; g < G; g++) {
+                if (g == 0) {
+                    drain(0, i);
+                    if (i + 2 < T) stage(bufA, 0);
+                    if (i + 3 < T) load(bufB, i + 3);
+                }
+                __syncthreads();
+            }
+        }
+        drain((int)((T - 1) & 1), T - 1);
+    } else {
+        __syncthreads();
+        for (uint64_t i = 0; i < T; i++) {
+            float *pre = planes[i & 1][0];
+            for (int g = 0; g < G; g++) {
+                Ops4<MODE> ops;
+                load_ops4<MODE>(ops, pa.b[g].mops, lane);
+                pass_apply4<TB, MODE, kPipeCompute>(pre, pa.b[g], ops, lane, wave, 1.0f, 1.0f);
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// Workgroups of the pipelined sweep: one per CU of the state's device.
+static unsigned pipe_grid(qsv_state *s)
+{
+    static int cus[64] = {0};
+    const int d = s->device >= 0 && s->device < 64 ? s->device : 0;
+    if (!cus[d]) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || c <= 0) c = 256;
+        cus[d] = c;
+    }
+    return (unsigned)cus[d];
 }
 
 // Tile bits a pass over these (widened) blocks needs: qubits 0..3 (the
@@ -377,8 +550,9 @@ template <int TB, bool ANY5>
 static void launch_tb5(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split)
 {
     float2 *a = (float2 *)s->amps;
-    const dim3 g((unsigned)ntiles), t(256);
-    if (split == 2) hipLaunchKernelGGL((k_fused_pass<TB, 2, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
+    const dim3 g((unsigned)ntiles), t(pass_threads(TB, ANY5));
+    if (split == 3) hipLaunchKernelGGL((k_fused_pass<TB, 3, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
+    else if (split == 2) hipLaunchKernelGGL((k_fused_pass<TB, 2, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
     else if (split) hipLaunchKernelGGL((k_fused_pass<TB, 1, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
     else hipLaunchKernelGGL((k_fused_pass<TB, 0, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
 }
@@ -393,14 +567,17 @@ static void launch_tb(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int spl
 }
 
 // blocks[g]: widened qubit list (4 or 5 qubits) and device operand pointer.
+// kernel: 0 automatic (one workgroup per tile), 1 one workgroup per tile,
+// 2 the pipelined sweep where it applies (64 KiB tiles of k = 4 blocks with
+// the BF16x6 or FP16x3 product).
 int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
-                    const std::vector<const float *> &mops, int split, int tb)
+                    const std::vector<const float *> &mops, int split, int tb, int kernel)
 {
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused passes need a c64 state");
     const int G = (int)qsets.size();
-    if (G < 1 || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..8 blocks per pass");
+    if (G < 1 || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..16 blocks per pass");
     if (tb < 11 || tb > kMaxPassTileBits || tb > s->n) return fail(QSV_ERR_ARG, "pass tile bits must be 11..13");
-    if (split < 0 || split > 2) return fail(QSV_ERR_ARG, "split must be 0, 1 or 2");
+    if (split < 0 || split > 3) return fail(QSV_ERR_ARG, "split must be 0, 1, 2 or 3");
     uint64_t u = 0;
     for (auto *q : qsets) {
         if (q->size() != 4 && q->size() != 5) return fail(QSV_ERR_ARG, "pass blocks must have 4 or 5 qubits");
@@ -413,6 +590,8 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     uint64_t lmask = u;
     for (int b = 0; __builtin_popcountll(lmask) < tb; b++) lmask |= uint64_t(1) << b;
     PassArgs pa{};
+    pa.bscale = s->f16_scale;
+    pa.binv = 1.0f / s->f16_scale;
     pa.tile_ins = insert_set_from_mask(lmask);
     const int *lpos = pa.tile_ins.pos;
     for (int j = 0; j < tb; j++) pa.lpos[j] = lpos[j];
@@ -423,7 +602,11 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
             if (e >> j & 1) o |= uint64_t(1) << lpos[j];
         return o;
     };
-    for (int i = 0; i < (1 << (tb - 9)); i++) pa.hioff[i] = pdep((uint64_t)512 * i);
+    bool any5 = false;
+    for (auto *q : qsets) any5 |= q->size() == 5;
+    const bool pipe = kernel == 2 && tb == 13 && !any5 && (split == 2 || split == 3);
+    const int nt = pipe ? kPipeMemThreads : pass_threads(tb, any5);   // threads that move tiles
+    for (int i = 0; i < (1 << tb) / (2 * nt); i++) pa.hioff[i] = pdep((uint64_t)(2 * nt) * i);
     auto rank = [&](int bit) {
         for (int i = 0; i < tb; i++)
             if (lpos[i] == bit) return i;
@@ -450,6 +633,15 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
         }
     }
     const uint64_t ntiles = s->dim >> tb;
+    if (pipe) {
+        const dim3 g((unsigned)std::min<uint64_t>(ntiles, pipe_grid(s))), t(kPipeThreads);
+        float2 *a = (float2 *)s->amps;
+        if (split == 3) hipLaunchKernelGGL(k_fused_pass_pipe<3>, g, t, 0, s->stream, a, ntiles, pa);
+        else hipLaunchKernelGGL(k_fused_pass_pipe<2>, g, t, 0, s->stream, a, ntiles, pa);
+        s->launches++;
+        QSV_HIP(hipGetLastError());
+        return QSV_OK;
+    }
     switch (tb) {
     case 11: launch_tb<11>(s, ntiles, pa, split); break;
     case 12: launch_tb<12>(s, ntiles, pa, split); break;

@@ -22,12 +22,19 @@ struct PlanItem {
 
 struct SchedChoice {
     int fuse_k = 4;
+    int window = 0;                // 1: the plan came from the window planner
     double pred_ms = 0;
 };
 
 int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
                 std::vector<PlanItem> &plan);
 void widen_to(PlanItem &it, int kq);
+
+// Window planner (qsv_window.cpp): blocks of <= max_k (4, 5) qubits grouped
+// into windows of tb tile bits; window[i] numbers the window of plan[i]
+// (barrier records get their own).
+int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
+                std::vector<PlanItem> &plan, std::vector<int> &window);
 
 // Learned scheduler (qsv_costmodel.hip): plans at every k, predicts each
 // block's time with the cost model, keeps the cheapest plan.
@@ -41,11 +48,11 @@ constexpr int kDefaultTileCols = 7;
 int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split, int tc);
 
 // Multi-block passes (qsv_pass.hip).
-constexpr int kMaxPassBlocks = 8;
+constexpr int kMaxPassBlocks = 16;
 constexpr int kMaxPassTileBits = 13;
 int pass_tile_bits(const std::vector<const std::vector<int> *> &qsets, int n);
 int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
-                    const std::vector<const float *> &mops, int split, int tb);
+                    const std::vector<const float *> &mops, int split, int tb, int kernel = 0);
 
 // One device launch of a fused run: plan items [first, first + count) (all
 // blocks) in one sweep.  count == 1 uses the single-block kernel with tile

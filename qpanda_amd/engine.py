@@ -26,7 +26,7 @@ EXPORTED = [
     "qsv_apply_block", "qsv_prob0", "qsv_norm2", "qsv_collapse", "qsv_measure",
     "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
     "qsv_sample_shots", "qsv_sample_shots_copy", "qsv_gather", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
-    "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan",
+    "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan", "qsv_window_plan",
     "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
     "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
     "qsv_timing_log", "qsv_batch_create", "qsv_batch_destroy", "qsv_batch_autotune", "qsv_batch_run",
@@ -44,7 +44,7 @@ class QsvError(RuntimeError):
 class RunOpts(C.Structure):
     _fields_ = [("exact", C.c_int), ("fuse_k", C.c_int), ("split", C.c_int),
                 ("scheduler", C.c_int), ("profile", C.c_int), ("tile", C.c_int),
-                ("group", C.c_int), ("pass_tb", C.c_int)]
+                ("group", C.c_int), ("pass_tb", C.c_int), ("pass_kernel", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -52,7 +52,8 @@ class RunStats(C.Structure):
                 ("tie_resolutions", C.c_int64), ("device_ms", C.c_double),
                 ("model_ms_pred", C.c_double), ("class_launches", C.c_int64 * 8),
                 ("class_ms", C.c_double * 8), ("class_bytes", C.c_double * 8),
-                ("class_flops", C.c_double * 8), ("plan_ms", C.c_double), ("fuse_k", C.c_int64)]
+                ("class_flops", C.c_double * 8), ("plan_ms", C.c_double), ("fuse_k", C.c_int64),
+                ("planner", C.c_int64)]
 
     CLASS_NAMES = ["dense1", "diag", "perm", "phase", "block64", "fused", "readout", "pass"]
 
@@ -130,6 +131,8 @@ def lib():
             "qsv_device_count": ([C.POINTER(I)], I),
             "qsv_fuse_plan": ([vp, C.c_int64, vp, I, I, C.c_int64, vp, vp, vp, vp,
                                C.POINTER(C.c_int64)], I),
+            "qsv_window_plan": ([vp, C.c_int64, vp, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
+                                 C.POINTER(C.c_int64)], I),
             "qsv_cm_create": ([U64, C.POINTER(vp)], I),
             "qsv_cm_destroy": ([vp], I),
             "qsv_cm_nparams": ([], I),
@@ -339,13 +342,16 @@ class StateVector:
         return out[:shots], rs.value, ties.value
 
     def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=2, scheduler=0,
-            rng_state=None, outcomes=None, profile=False, tile=0, group=0, pass_tb=0) -> RunStats:
+            rng_state=None, outcomes=None, profile=False, tile=0, group=0, pass_tb=0,
+            pass_kernel=0) -> RunStats:
         """scheduler: 0 greedy blocks (one sweep each), 1 learned (needs an attached
-        cost model), 2 greedy blocks grouped into multi-block sweeps of <= group."""
+        cost model), 2 greedy blocks grouped into multi-block sweeps of <= group,
+        3 window planner (one sweep per window of pass_tb tile bits, 0 = 13).
+        pass_kernel: 0 automatic, 1 one workgroup per tile, 2 pipelined sweeps."""
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
         opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile),
-                       int(group), int(pass_tb))
+                       int(group), int(pass_tb), int(pass_kernel))
         st = RunStats()
         rs = C.c_int32(rng_state if rng_state is not None else 1)
         self._last_rng = None
@@ -534,6 +540,36 @@ def fuse_plan(circ: "_circ.Circuit", max_k: int, exact: bool = False):
         d = 1 << k
         u = mats[b * 2048: b * 2048 + 2 * d * d].view(np.complex128).reshape(d, d).copy()
         out.append((qs[b * 5: b * 5 + k].tolist(), u, int(ngs[b])))
+    return out
+
+
+def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = False):
+    """Window plan (host code; no device needed): list of (qubits, matrix,
+    ngates, window) blocks, or (None, None, record_index, window) for
+    MEASURE/RESET records, in emission order."""
+    g = np.ascontiguousarray(circ.gates)
+    m = np.ascontiguousarray(circ.mats, np.complex128)
+    cap = max(1, g.shape[0])
+    ks = np.zeros(cap, np.int32)
+    qs = np.zeros(cap * 5, np.int32)
+    mats = np.zeros(cap * 2048, np.float64)
+    ngs = np.zeros(cap, np.int64)
+    win = np.zeros(cap, np.int32)
+    nb = C.c_int64()
+    _check(lib().qsv_window_plan(g.ctypes.data_as(C.c_void_p), g.shape[0],
+                                 m.ctypes.data_as(C.c_void_p) if m.size else None, max_k, int(exact), tb,
+                                 circ.n, cap, ks.ctypes.data_as(C.c_void_p), qs.ctypes.data_as(C.c_void_p),
+                                 mats.ctypes.data_as(C.c_void_p), ngs.ctypes.data_as(C.c_void_p),
+                                 win.ctypes.data_as(C.c_void_p), C.byref(nb)))
+    out = []
+    for b in range(nb.value):
+        k = int(ks[b])
+        if k == 0:
+            out.append((None, None, int(ngs[b]), int(win[b])))
+            continue
+        d = 1 << k
+        u = mats[b * 2048: b * 2048 + 2 * d * d].view(np.complex128).reshape(d, d).copy()
+        out.append((qs[b * 5: b * 5 + k].tolist(), u, int(ngs[b]), int(win[b])))
     return out
 
 

@@ -75,3 +75,97 @@ def test_plan_blocks_are_unitary():
     for qs, u, _ in E.fuse_plan(circ, 5):
         d = u.shape[0]
         assert np.abs(u.conj().T @ u - np.eye(d)).max() < 1e-12
+
+
+# ---- window planner (qsv_window.cpp) ------------------------------------------
+
+def apply_window_plan(plan, n, psi, exact=False):
+    st = O.State(n, exact=exact)
+    st.set(psi)
+    for qs, u, _, _ in plan:
+        assert qs is not None
+        st.block(qs, u)
+    return st
+
+
+@pytest.mark.parametrize("k", [4, 5])
+@pytest.mark.parametrize("tb", [6, 8, 13])
+def test_window_plan_reproduces_random_circuit(k, tb):
+    n = 11
+    circ = C.random_layers(n, 10, 300 + k + tb, haar_frac=0.2)
+    psi = C.random_gaussian_state(n, k + tb)
+    ref = O.State(n)
+    ref.set(psi)
+    ref.run(circ)
+    plan = E.window_plan(circ, k, tb)
+    assert sum(ng for _, _, ng, _ in plan) == circ.ngates
+    assert all(len(qs) <= k for qs, _, _, _ in plan)
+    # every window's blocks fit in tb qubits together with qubits 0..3
+    wins = {}
+    for qs, _, _, w in plan:
+        wins.setdefault(w, set(range(4))).update(qs)
+    assert all(len(s) <= max(tb, 4) for s in wins.values())
+    got = apply_window_plan(plan, n, psi)
+    assert np.abs(got.psi - ref.psi).max() < 1e-12
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_window_plan_every_gate_kind(exact):
+    n = 9
+    circ = C.all_gate_kinds(n, 21, reps=4)
+    psi = C.random_gaussian_state(n, 77)
+    ref = O.State(n, exact=exact)
+    ref.set(psi)
+    ref.run(circ)
+    plan = E.window_plan(circ, 4, 7, exact=exact)
+    assert sum(ng for _, _, ng, _ in plan) == circ.ngates
+    got = apply_window_plan(plan, n, psi, exact=exact)
+    assert np.abs(got.psi - ref.psi).max() < 1e-12
+
+
+def test_window_plan_counts_noop_records():
+    b = C._Builder(12)
+    b.add(C.H, 5)
+    b.add(C.CNOT, 7, 7)          # a no-op in the reference loop, but one draw
+    b.add(C.CNOT, 5, 9)
+    circ = b.build()
+    plan = E.window_plan(circ, 4, 13)
+    assert sum(ng for _, _, ng, _ in plan) == 3
+
+
+def test_window_plan_keeps_measurements_in_order():
+    b = C._Builder(12)
+    for q in range(12):
+        b.add(C.H, q)
+    b.add(C.MEASURE, 7, 0)
+    b.add(C.CNOT, 7, 11)
+    b.add(C.RESET, 4)
+    b.add(C.RX, 4, theta=0.4)
+    circ = b.build()
+    plan = E.window_plan(circ, 4, 13)
+    assert [ng for qs, _, ng, _ in plan if qs is None] == [12, 14]
+    first = [i for i, p in enumerate(plan) if p[0] is None][0]
+    assert sum(p[2] for p in plan[:first]) == 12
+    # barriers sit in windows of their own
+    wb = [p[3] for p in plan if p[0] is None]
+    assert all(sum(1 for p in plan if p[3] == w) == 1 for w in wb)
+
+
+def test_window_plan_needs_fewer_sweeps_than_block_first_grouping():
+    """30 qubits, depth 200 (the benchmark circuit): the window planner needs
+    at most 300 sweeps of 13 tile bits for its 9,000 gates; block-first
+    grouping of greedy k = 4 blocks (every run of consecutive blocks whose
+    qubits above 3 number <= 9 shares a sweep) needs more than 400."""
+    circ = C.random_layers(30, 200, 2025, haar_frac=0.1)
+    plan = E.window_plan(circ, 4, 13)
+    windows = len({w for *_, w in plan})
+    blocks = E.fuse_plan(circ, 4)
+    sweeps, cur = 0, None
+    for qs, _, _ in blocks:
+        hi = {q for q in qs if q > 3}
+        if cur is not None and len(cur | hi) <= 9:
+            cur |= hi
+        else:
+            sweeps += 1
+            cur = set(hi)
+    assert windows <= 300 < 400 < sweeps, (windows, sweeps)

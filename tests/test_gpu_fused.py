@@ -1,6 +1,8 @@
 """Fused FP32 MFMA blocks (complex64 state) against the FP64 oracle, for the
-three products: FP32 (split 0), FP32 + BF16 lo correction (split 1) and BF16x6
-(split 2, the product on the BF16 matrix cores).
+four products: FP32 (split 0), FP32 + BF16 lo correction (split 1), BF16x6
+(split 2, the product on the BF16 matrix cores) and FP16x3 (split 3, two fp16
+pieces of matrix and scaled state, three piece products on the FP16 matrix
+cores).
 
 Bars: one block application within 2e-6 max amplitude error (FP32 storage and
 accumulation, ~1e-7 relative); whole fused circuits at fidelity >= 1 - 1e-6
@@ -35,7 +37,7 @@ QSETS = {
 
 
 @pytest.mark.parametrize("k", [2, 3, 4, 5])
-@pytest.mark.parametrize("split", [0, 1, 2])
+@pytest.mark.parametrize("split", [0, 1, 2, 3])
 def test_block_mfma_vs_oracle(k, split):
     n = 16
     for idx, qs in enumerate(QSETS[k]):
@@ -65,19 +67,43 @@ def test_split_is_not_less_accurate():
     for _ in range(20):
         st.block(qs, u)
     errs = {}
-    for split in (0, 1, 2):
+    for split in (0, 1, 2, 3):
         with E.StateVector(n, E.C64) as sv:
             sv.set(psi)
             for _ in range(20):
                 sv.block(qs, u, split=split)
             errs[split] = np.abs(sv.get() - st.psi).max()
-    say("20 blocks, max error FP32 / split / BF16x6", errs[0], errs[1], errs[2])
+    say("20 blocks, max error FP32 / split / BF16x6 / FP16x3", errs[0], errs[1], errs[2], errs[3])
     assert errs[1] <= errs[0] * 1.1
     assert errs[2] <= errs[0] * 1.1
+    # FP16x3 keeps 22 of the 24 operand bits: within 8x of the FP32 product
+    assert errs[3] <= errs[0] * 8
+
+
+@pytest.mark.parametrize("scale", [1e-7, 1.0, 300.0])
+def test_fp16x3_scale_follows_the_norm(scale):
+    """FP16x3 scales the state by 2^(15 - e), 2^e >= sqrt(norm): unnormalised
+    states (norm 1e-14 .. 9e4) keep the relative accuracy of a normalised one."""
+    n = 16
+    qs = [0, 5, 11, 15]
+    u = C.haar_unitary(C.Xoshiro256(5), 16)
+    psi = C.random_gaussian_state(n, 11) * scale
+    psi[12345] = 0.9 * scale          # one large amplitude next to ~2^-8 ones
+    st = O.State(n)
+    st.set(psi)
+    st.block(qs, u)
+    with E.StateVector(n, E.C64) as sv:
+        sv.set(psi)
+        sv.block(qs, u, split=3)
+        got = sv.get()
+    rel = np.abs(got - st.psi).max() / np.abs(st.psi).max()
+    say("fp16x3 scale", scale, "relative max error", rel)
+    assert np.all(np.isfinite(got))
+    assert rel < 2e-6
 
 
 @pytest.mark.parametrize("k", [2, 3, 4, 5])
-@pytest.mark.parametrize("split", [1, 2])
+@pytest.mark.parametrize("split", [1, 2, 3])
 def test_fused_circuit_fidelity(k, split):
     n = 16
     circ = C.random_layers(n, 20, 500 + k, haar_frac=0.1)
@@ -143,7 +169,7 @@ def test_fused_large_state_sweep_30q():
 
 @pytest.mark.parametrize("group,pass_tb", [(2, 0), (3, 0), (8, 0), (4, 13)])
 @pytest.mark.parametrize("k", [4, 5])
-@pytest.mark.parametrize("split", [1, 2])
+@pytest.mark.parametrize("split", [1, 2, 3])
 def test_multi_block_passes_vs_oracle(group, pass_tb, k, split):
     n = 18
     circ = C.random_layers(n, 16, 900 + group + k, haar_frac=0.1)
@@ -179,3 +205,69 @@ def test_passes_with_measurements_keep_rng_accounting():
         f.run(c2, fuse_k=4, scheduler=2, group=8, rng_state=99)
         p.run(c2, rng_state=99)
         assert f.last_rng == p.last_rng
+
+
+@pytest.mark.parametrize("pass_tb", [0, 12])
+@pytest.mark.parametrize("k", [4, 5])
+@pytest.mark.parametrize("split", [2, 3])
+@pytest.mark.parametrize("pass_kernel", [1, 2])
+def test_window_scheduler_vs_oracle(pass_tb, k, split, pass_kernel):
+    """Window planner (scheduler 3): one sweep per window, every block inside
+    it; the state matches the FP64 oracle at the fused-path bar, with the
+    one-workgroup-per-tile sweep (1) and the pipelined one (2, used for
+    13-bit windows of k = 4 blocks)."""
+    n = 18
+    circ = C.random_layers(n, 16, 700 + k + pass_tb, haar_frac=0.1)
+    ref, _ = O.run_circuit(circ)
+    say("window scheduler tb", pass_tb, "k", k, "split", split, "kernel", pass_kernel)
+    with E.StateVector(n, E.C64) as sv:
+        st = sv.run(circ, fuse_k=k, split=split, scheduler=3, pass_tb=pass_tb, pass_kernel=pass_kernel,
+                    profile=True)
+        got = sv.get()
+    assert st.planner == 1 and st.fuse_k == k
+    windows = len({w for *_, w in E.window_plan(circ, k, pass_tb or 13)})
+    cl = st.classes()
+    sweeps = cl.get("fused", {}).get("launches", 0) + cl.get("pass", {}).get("launches", 0)
+    assert windows <= sweeps <= windows + st.blocks // 16
+    fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
+    say("sweeps", sweeps, "blocks", st.blocks, "fidelity", fid)
+    assert fid >= 1 - 1e-6
+    assert np.abs(got - ref.psi).max() < 1e-5
+
+
+def test_window_scheduler_with_measurements_keeps_rng_accounting():
+    n = 14
+    b = C._Builder(n)
+    for g in C.random_layers(n, 6, 15).gates:
+        b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    b.add(C.CNOT, 6, 6)
+    b.add(C.MEASURE, 9, 0)
+    b.add(C.RESET, 2)
+    for g in C.random_layers(n, 3, 16).gates:
+        b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    circ = b.build()
+    out_f = np.zeros(8, np.int32)
+    out_p = np.zeros(8, np.int32)
+    with E.StateVector(n, E.C64) as f, E.StateVector(n, E.C128) as p:
+        f.run(circ, fuse_k=4, split=3, scheduler=3, rng_state=321, outcomes=out_f)
+        p.run(circ, rng_state=321, outcomes=out_p)
+        assert f.last_rng == p.last_rng          # one draw per record, as the reference
+        if out_f[0] == out_p[0]:
+            assert f.fidelity(p) >= 1 - 1e-6
+
+
+@pytest.mark.parametrize("n", [13, 14, 22])
+@pytest.mark.parametrize("split", [2, 3])
+def test_pipelined_sweep_matches_tile_sweep(n, split):
+    """The pipelined sweep (persistent workgroups, one to every CU, tiles in
+    turn) and the one-workgroup-per-tile sweep give the same state: one tile
+    (n = 13), fewer tiles than CUs (14) and several tiles per workgroup (22)."""
+    circ = C.random_layers(n, 8, 4400 + n, haar_frac=0.1)
+    outs = []
+    for kern in (1, 2):
+        with E.StateVector(n, E.C64) as sv:
+            st = sv.run(circ, fuse_k=4, split=split, scheduler=3, pass_kernel=kern, profile=True)
+            outs.append(sv.get())
+        assert st.classes().get("pass", {}).get("launches", 0) > 0
+    say("pipelined vs tile sweep n", n, "max diff", np.abs(outs[0] - outs[1]).max())
+    assert np.abs(outs[0] - outs[1]).max() < 1e-6
