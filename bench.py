@@ -9,12 +9,14 @@ two-qubit unitaries), 9,000 logical gates, synthetic, generated from a fixed see
 Paths:
   plain  per-gate complex128 sweeps (one launch per gate), the FP64 reference path;
   fused  complex64 state, gates fused into <= k-qubit blocks run as complex
-         GEMMs on the matrix cores -- by default the BF16x6 product (matrix and
-         state split into three bf16 pieces, six piece products accumulated in
-         FP32; --product split: FP32 MFMA plus a BF16 correction of the
-         matrix's lo part; --product fp32) -- scheduled by the trained cost
-         model (block size, grouping of blocks into shared HBM sweeps, tile
-         shape); --scheduler greedy gives one sweep per block.
+         GEMMs on the matrix cores -- by default the FP16x3 product (matrix and
+         power-of-two-scaled state split into fp16 hi + lo pieces, three piece
+         products accumulated in FP32; --product bf16x6: three bf16 pieces,
+         six products; --product split: FP32 MFMA plus a BF16 correction of
+         the matrix's lo part; --product fp32) -- scheduled by the trained cost
+         model (planner: block-first or 13-qubit windows; block size, grouping
+         of blocks into shared HBM sweeps, tile shape); --scheduler greedy
+         gives one sweep per block, --scheduler window one sweep per window.
 A step is one full circuit execution from |0...0> on the headline path (fused).
 The plain path and the greedy k=4 fused path each run once for comparison;
 fidelity |<plain|fused>|^2 is computed on the device in FP64.
@@ -44,10 +46,11 @@ HIP events on the state's stream inside the timed steps; achieved = algorithmic
 bytes (every amplitude a gate may change, read and written once) / time.  A
 fused k=4 block performs 8 complex-GEMM flop per byte moved; a multi-block
 sweep ("pass") 8 flop/B per block.  The matrix-core peak in those flops is
-the FP32 MFMA peak (157.3 TF, ridge ~20 flop/B) for the FP32 products and the
+the FP32 MFMA peak (157.3 TF, ridge ~20 flop/B) for the FP32 products, the
 BF16 dense peak / 6 (417 TF, ridge ~52 flop/B) for BF16x6, which executes
-six bf16 piece products per flop; a sweep is MFMA-bound once its intensity
-crosses the ridge, else HBM-bound.  The matrix-core rate is always reported
+six bf16 piece products per flop, and the FP16 dense peak / 3 (833 TF, ridge
+~104 flop/B) for FP16x3; a sweep is MFMA-bound once its intensity crosses the
+ridge, else HBM-bound.  The matrix-core rate is always reported
 ("mfma").  "copy_gbs" is a
 state-to-state copy of the same size timed in the same run: the practical
 streaming rate of the device, for reference.
@@ -97,7 +100,7 @@ def parse():
                    help="tile bits of multi-block sweeps (passes, window; 0 = minimal / 13)")
     p.add_argument("--costmodel", default=None, help="trained cost-model parameters (float32)")
     p.add_argument("--path", choices=["fused", "plain"], default="fused")
-    p.add_argument("--product", choices=["fp32", "split", "bf16x6", "fp16x3"], default="bf16x6",
+    p.add_argument("--product", choices=["fp32", "split", "bf16x6", "fp16x3"], default="fp16x3",
                    help="matrix product of the fused blocks")
     p.add_argument("--no-plain", action="store_true", help="skip the FP64 and greedy comparison runs")
     p.add_argument("--no-cpu", action="store_true")

@@ -20,6 +20,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <string>
+#include <thread>
 
 struct qsv_costmodel {
     float *params = nullptr;     // device, kParams
@@ -232,37 +234,44 @@ static int default_tc(int n, int kq)
 }
 
 int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_run_opts &o,
-                   std::vector<Launch> &launches, double *pred_total)
+                   std::vector<Launch> &launches, double *pred_total, const std::vector<int> *window)
 {
     launches.clear();
     *pred_total = 0;
     const int n = s->n;
     const bool learned = o.scheduler == 1 && s->cm;
     const int gmax = learned ? kMaxPassBlocks : std::max(1, std::min(o.group, kMaxPassBlocks));
-    size_t i = 0;
-    while (i < plan.size()) {
-        if (plan[i].barrier >= 0) {
-            Launch L;
-            L.first = i;
-            launches.push_back(L);
-            i++;
-            continue;
-        }
-        // segment of consecutive blocks up to the next barrier
-        size_t j = i;
-        while (j < plan.size() && plan[j].barrier < 0) j++;
-        const size_t m = j - i;
-        auto mfma_ok = [&](size_t b) { return plan[b].qubits.size() >= 4 && (int)plan[b].qubits.size() + 6 <= n; };
-        if (!learned) {
-            size_t b = i;
-            while (b < j) {
+    auto mfma_ok = [&](size_t b) { return plan[b].qubits.size() >= 4 && (int)plan[b].qubits.size() + 6 <= n; };
+    // segments: runs of consecutive blocks up to the next barrier (and, for a
+    // window plan, the end of the window: its blocks are planned as one sweep,
+    // and the DP only chooses how to launch them); barriers are segments of
+    // their own with first == last
+    struct Seg { size_t i, j; size_t c0 = 0, c1 = 0; };
+    std::vector<Seg> segs;
+    for (size_t i = 0; i < plan.size();) {
+        size_t j = i + 1;
+        if (plan[i].barrier < 0)
+            while (j < plan.size() && plan[j].barrier < 0 && (!window || (*window)[j] == (*window)[i])) j++;
+        segs.push_back({i, j});
+        i = j;
+    }
+    if (!learned) {
+        for (const Seg &sg : segs) {
+            if (plan[sg.i].barrier >= 0) {
+                Launch L;
+                L.first = sg.i;
+                launches.push_back(L);
+                continue;
+            }
+            size_t b = sg.i;
+            while (b < sg.j) {
                 Launch L;
                 L.first = b;
                 L.count = 1;
                 L.tc = mfma_ok(b) ? (o.tile ? o.tile : default_tc(n, (int)plan[b].qubits.size())) : 0;
                 if (mfma_ok(b) && (int)plan[b].qubits.size() + L.tc > n) L.tc = default_tc(n, (int)plan[b].qubits.size());
                 std::vector<const std::vector<int> *> q = {&plan[b].qubits};
-                while (mfma_ok(b) && gmax > 1 && b + L.count < j && L.count < gmax && mfma_ok(b + L.count)) {
+                while (mfma_ok(b) && gmax > 1 && b + L.count < sg.j && L.count < gmax && mfma_ok(b + L.count)) {
                     q.push_back(&plan[b + L.count].qubits);
                     int tb = pass_tile_bits(q, n);
                     if (tb < 0) {
@@ -278,142 +287,193 @@ int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_ru
                 launches.push_back(L);
                 b += L.count;
             }
-            i = j;
-            continue;
         }
-        // learned: dynamic programming over the segment with predicted times
-        struct Cand { size_t start; int count, tc, tb; };
-        std::vector<Cand> cands;
-        std::vector<float> feats;
-        for (size_t e = 0; e < m; e++) {            // candidate launches ending at block i+e
-            const size_t last = i + e;
-            if (!mfma_ok(last)) {
-                cands.push_back({last, 1, 0, 0});
-                float f[F] = {0};
-                feats.insert(feats.end(), f, f + F);
-                continue;
-            }
-            const int kq = (int)plan[last].qubits.size();
-            std::vector<const std::vector<int> *> q1 = {&plan[last].qubits};
-            for (int tc = 6; tc <= 8; tc++) {
-                if (kq + tc > n) continue;
-                float f[F];
-                launch_features(n, o.split, q1, true, kq + tc, f);
-                cands.push_back({last, 1, tc, 0});
-                feats.insert(feats.end(), f, f + F);
-            }
-            std::vector<const std::vector<int> *> q = q1;
-            for (int g = 2; g <= kMaxPassBlocks && (size_t)g <= e + 1; g++) {
-                const size_t st = last + 1 - g;
-                if (!mfma_ok(st)) break;
-                q.insert(q.begin(), &plan[st].qubits);
-                const int tbmin = pass_tile_bits(q, n);
-                if (tbmin < 0) break;
-                for (int tb = tbmin; tb <= std::min(kMaxPassTileBits, n); tb++) {
+        return QSV_OK;
+    }
+    // learned: every candidate launch of every segment, predicted in one
+    // batched inference, then a dynamic program per segment
+    struct Cand { size_t start; int count, tc, tb; };
+    std::vector<Cand> cands;
+    std::vector<float> feats;
+    for (Seg &sg : segs) {
+        sg.c0 = cands.size();
+        if (plan[sg.i].barrier < 0)
+            for (size_t last = sg.i; last < sg.j; last++) {     // candidate launches ending at block `last`
+                if (!mfma_ok(last)) {
+                    cands.push_back({last, 1, 0, 0});
+                    float f[F] = {0};
+                    feats.insert(feats.end(), f, f + F);
+                    continue;
+                }
+                const int kq = (int)plan[last].qubits.size();
+                std::vector<const std::vector<int> *> q1 = {&plan[last].qubits};
+                for (int tc = 6; tc <= 8; tc++) {
+                    if (kq + tc > n) continue;
                     float f[F];
-                    launch_features(n, o.split, q, false, tb, f);
-                    cands.push_back({st, g, 0, tb});
+                    launch_features(n, o.split, q1, true, kq + tc, f);
+                    cands.push_back({last, 1, tc, 0});
                     feats.insert(feats.end(), f, f + F);
                 }
+                std::vector<const std::vector<int> *> q = q1;
+                for (int g = 2; g <= kMaxPassBlocks && (size_t)g <= last - sg.i + 1; g++) {
+                    const size_t st = last + 1 - g;
+                    if (!mfma_ok(st)) break;
+                    q.insert(q.begin(), &plan[st].qubits);
+                    const int tbmin = pass_tile_bits(q, n);
+                    if (tbmin < 0) break;
+                    for (int tb = tbmin; tb <= std::min(kMaxPassTileBits, n); tb++) {
+                        float f[F];
+                        launch_features(n, o.split, q, false, tb, f);
+                        cands.push_back({st, g, 0, tb});
+                        feats.insert(feats.end(), f, f + F);
+                    }
+                }
             }
-        }
-        std::vector<float> y(cands.size());
+        sg.c1 = cands.size();
+    }
+    std::vector<float> y(cands.size());
+    if (!cands.empty()) {
         int rc = qsv_cm_forward(s->cm, feats.data(), (int64_t)cands.size(), y.data());
         if (rc) return rc;
-        const double scale = std::ldexp(1.0, n - 30);
+    }
+    const double scale = std::ldexp(1.0, n - 30);
+    auto cost = [&](size_t c) {
+        const Cand &cd = cands[c];
+        return (cd.tc == 0 && cd.tb == 0) ? 0.0 : std::exp((double)y[c]) * scale;
+    };
+    // a window keeps its one sweep (at its minimal tile bits, or the default
+    // tile shape for a lone block) unless the DP predicts a gain above the
+    // model's resolution: deviations on smaller predicted gains measured
+    // slower more often than faster
+    constexpr double kWindowMargin = 0.05;     // ~ the per-launch prediction error
+    for (const Seg &sg : segs) {
+        const size_t i = sg.i, m = sg.j - sg.i;
+        if (plan[i].barrier >= 0) {
+            Launch L;
+            L.first = i;
+            launches.push_back(L);
+            continue;
+        }
         std::vector<double> dp(m + 1, std::numeric_limits<double>::infinity());
-        std::vector<int> from(m + 1, -1);
+        std::vector<size_t> from(m + 1, 0);
         dp[0] = 0;
-        // candidates are grouped by their end block, in increasing order
-        for (size_t c = 0; c < cands.size(); c++) {
+        // candidates come grouped by their end block, in increasing order
+        for (size_t c = sg.c0; c < sg.c1; c++) {
             const Cand &cd = cands[c];
             const size_t end = cd.start + cd.count - i;      // dp index after this launch
             const size_t beg = cd.start - i;
-            const double ms = (cd.tc == 0 && cd.tb == 0) ? 0.0 : std::exp((double)y[c]) * scale;
+            const double ms = cost(c);
             if (dp[beg] + ms < dp[end]) {
                 dp[end] = dp[beg] + ms;
-                from[end] = (int)c;
+                from[end] = c;
             }
         }
+        long whole = -1;
+        if (window)
+            for (size_t c = sg.c0; c < sg.c1; c++) {
+                const Cand &cd = cands[c];
+                if (cd.start != i || (size_t)cd.count != m) continue;
+                const bool lone = m == 1 && cd.tc == default_tc(n, (int)plan[i].qubits.size());
+                const bool pass = m > 1 && (whole < 0 || cd.tb < cands[whole].tb);
+                if (lone || pass || (m == 1 && cd.tc == 0 && cd.tb == 0)) whole = (long)c;
+            }
         std::vector<Launch> seg;
-        for (size_t e = m; e > 0;) {
-            const Cand &cd = cands[from[e]];
+        if (whole >= 0 && dp[m] > cost(whole) * (1 - kWindowMargin)) {
+            const Cand &cd = cands[whole];
             Launch L;
             L.first = cd.start;
             L.count = cd.count;
             L.tc = cd.tc;
             L.tb = cd.tb;
-            L.pred_ms = dp[e] - dp[cd.start - i];
+            L.pred_ms = cost(whole);
             seg.push_back(L);
-            e = cd.start - i;
+            *pred_total += L.pred_ms;
+        } else {
+            for (size_t e = m; e > 0;) {
+                const Cand &cd = cands[from[e]];
+                Launch L;
+                L.first = cd.start;
+                L.count = cd.count;
+                L.tc = cd.tc;
+                L.tb = cd.tb;
+                L.pred_ms = dp[e] - dp[cd.start - i];
+                seg.push_back(L);
+                e = cd.start - i;
+            }
+            std::reverse(seg.begin(), seg.end());
+            *pred_total += dp[m];
         }
-        std::reverse(seg.begin(), seg.end());
         launches.insert(launches.end(), seg.begin(), seg.end());
-        *pred_total += dp[m];
-        i = j;
     }
     return QSV_OK;
 }
 
 int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *mats, const qsv_run_opts &o,
-                 std::vector<PlanItem> &plan, SchedChoice *choice)
+                 std::vector<PlanItem> &plan, SchedChoice *choice, std::vector<int> *window)
 {
+    window->clear();
     if (!s->cm || s->n < 4 + 6) {
         choice->fuse_k = o.fuse_k;
         return plan_greedy(gates, ng, mats, o.fuse_k, o.exact, plan);
     }
-    double best_total = std::numeric_limits<double>::infinity();
-    int best_k = o.fuse_k;
-    std::vector<PlanItem> best;
+    // candidate plans: block-first at k = 2..5 and window plans at k = 4, 5
+    // (states of >= 11 qubits).  Planning is host work with no shared state,
+    // so the candidates are built on threads of their own; the cost-model DP
+    // (device inference on the model's stream) then scores them in turn.
+    struct Cand {
+        int k = 0, window = 0, rc = QSV_OK;
+        std::string err;                 // the planner's message (errors are per thread)
+        std::vector<PlanItem> plan;
+        std::vector<int> win;
+    };
+    std::vector<Cand> cands;
     for (int k = 2; k <= 5; k++) {
-        std::vector<PlanItem> cand;
-        int rc = plan_greedy(gates, ng, mats, k, o.exact, cand);
-        if (rc) return rc;
-        for (PlanItem &it : cand)
-            if (it.barrier < 0 && s->n >= std::max(4, (int)it.qubits.size()) + 6) {
-                // qubit list as the run will widen it (the matrix is widened later)
-                it.qubits = widened_qubits(it.qubits, std::max(4, (int)it.qubits.size()));
-            }
+        Cand c;
+        c.k = k;
+        cands.push_back(std::move(c));
+    }
+    if (s->n >= 11)
+        for (int k = 4; k <= 5; k++) {
+            Cand c;
+            c.k = k;
+            c.window = 1;
+            cands.push_back(std::move(c));
+        }
+    std::vector<std::thread> pool;
+    for (Cand &c : cands)
+        pool.emplace_back([&c, gates, ng, mats, &o, s]() {
+            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win)
+                            : plan_greedy(gates, ng, mats, c.k, o.exact, c.plan);
+            if (c.rc) c.err = qsv_last_error();
+        });
+    for (std::thread &t : pool) t.join();
+    double best_total = std::numeric_limits<double>::infinity();
+    Cand *best = nullptr;
+    for (Cand &c : cands) {
+        if (c.rc) return fail(c.rc, c.err);
+        // the DP sees the qubit lists as the run will widen them (the
+        // matrices are widened later); it needs nothing else of a block
+        std::vector<PlanItem> shape(c.plan.size());
+        for (size_t i = 0; i < c.plan.size(); i++) {
+            shape[i].barrier = c.plan[i].barrier;
+            shape[i].qubits = c.plan[i].qubits;
+            if (c.plan[i].barrier < 0 && s->n >= std::max(4, (int)c.plan[i].qubits.size()) + 6)
+                shape[i].qubits = widened_qubits(c.plan[i].qubits, std::max(4, (int)c.plan[i].qubits.size()));
+        }
         std::vector<Launch> launches;
         double total = 0;
-        rc = group_launches(s, cand, o, launches, &total);
+        int rc = group_launches(s, shape, o, launches, &total, c.window ? &c.win : nullptr);
         if (rc) return rc;
         if (total < best_total) {
             best_total = total;
-            best_k = k;
-            // re-plan to keep the matrices consistent with their qubit lists
-            best.clear();
-            rc = plan_greedy(gates, ng, mats, k, o.exact, best);
-            if (rc) return rc;
+            best = &c;
         }
     }
-    // window plans (qsv_window.cpp) at k = 4, 5, grouped by the same DP
-    int best_window = 0;
-    if (s->n >= 11) {
-        for (int k = 4; k <= 5; k++) {
-            std::vector<PlanItem> cand;
-            std::vector<int> win;
-            int rc = plan_window(gates, ng, mats, k, o.exact, kMaxPassTileBits, s->n, cand, win);
-            if (rc) return rc;
-            for (PlanItem &it : cand)
-                if (it.barrier < 0) it.qubits = widened_qubits(it.qubits, std::max(4, (int)it.qubits.size()));
-            std::vector<Launch> launches;
-            double total = 0;
-            rc = group_launches(s, cand, o, launches, &total);
-            if (rc) return rc;
-            if (total < best_total) {
-                best_total = total;
-                best_k = k;
-                best_window = 1;
-                best.clear();
-                rc = plan_window(gates, ng, mats, k, o.exact, kMaxPassTileBits, s->n, best, win);
-                if (rc) return rc;
-            }
-        }
-    }
-    plan.swap(best);
-    choice->fuse_k = best_k;
-    choice->window = best_window;
+    if (!best) return fail(QSV_ERR_NUMERIC, "cost model predicted no finite schedule");
+    plan.swap(best->plan);
+    window->swap(best->win);
+    choice->fuse_k = best->k;
+    choice->window = best->window;
     choice->pred_ms = best_total;
     return QSV_OK;
 }

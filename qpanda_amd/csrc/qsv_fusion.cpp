@@ -22,6 +22,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <thread>
 
 namespace qsv {
 
@@ -338,7 +339,7 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
     if (o.pass_tb != 0 && (o.pass_tb < 11 || o.pass_tb > kMaxPassTileBits))
         return fail(QSV_ERR_ARG, "pass tile bits must be 0 or 11..13");
     if (o.scheduler == 1) {
-        rc = plan_learned(s, gates, ngates, mats, o, plan, &choice);
+        rc = plan_learned(s, gates, ngates, mats, o, plan, &choice, &window);
     } else if (windowed) {
         choice.fuse_k = std::max(4, std::min(o.fuse_k, 5));
         choice.window = 1;
@@ -363,7 +364,7 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
     std::vector<Launch> launches;
     double pred = 0;
     if (o.scheduler == 1 && s->cm) {
-        rc = group_launches(s, plan, o, launches, &pred);
+        rc = group_launches(s, plan, o, launches, &pred, window.empty() ? nullptr : &window);
     } else if (windowed) {
         // one sweep per window (several beyond the pass kernel's block limit)
         const int tb_min = o.pass_tb ? o.pass_tb : 11;
@@ -394,15 +395,26 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
         rc = group_launches(s, plan, g, launches, &pred);
     }
     if (rc) return rc;
-    // operands for every MFMA block in one upload
+    // operands for every MFMA block in one upload, built on a few threads
     std::vector<size_t> op_off(plan.size(), 0);
-    std::vector<float> ops;
+    size_t op_total = 0;
     for (size_t i = 0; i < plan.size(); i++) {
         if (!mfma[i]) continue;
-        const int kq = (int)plan[i].qubits.size();
-        op_off[i] = ops.size();
-        ops.resize(ops.size() + fused_operand_floats(kq, o.split));
-        fused_operands(kq, plan[i].u.data(), o.split, ops.data() + op_off[i]);
+        op_off[i] = op_total;
+        op_total += fused_operand_floats((int)plan[i].qubits.size(), o.split);
+    }
+    std::vector<float> ops(op_total);
+    {
+        const size_t nth = std::max<size_t>(1, std::min<size_t>({8, std::thread::hardware_concurrency(),
+                                                                  plan.size() / 64 + 1}));
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nth; t++)
+            pool.emplace_back([&, t]() {
+                for (size_t i = t; i < plan.size(); i += nth)
+                    if (mfma[i])
+                        fused_operands((int)plan[i].qubits.size(), plan[i].u.data(), o.split, ops.data() + op_off[i]);
+            });
+        for (std::thread &th : pool) th.join();
     }
     float *dops = nullptr;
     if (!ops.empty()) {

@@ -39,7 +39,8 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
 // Learned scheduler (qsv_costmodel.hip): plans at every k, predicts each
 // block's time with the cost model, keeps the cheapest plan.
 int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *mats,
-                 const qsv_run_opts &o, std::vector<PlanItem> &plan, SchedChoice *choice);
+                 const qsv_run_opts &o, std::vector<PlanItem> &plan, SchedChoice *choice,
+                 std::vector<int> *window);
 
 // Fused MFMA kernel helpers (qsv_fused.hip).
 void fused_operands(int kq, const double *u, int split, float *out);
@@ -72,6 +73,6 @@ void launch_features(int n, int split, const std::vector<const std::vector<int> 
 // Group a block plan into launches.  scheduler 0: one block per launch;
 // 2: greedy maximal groups (opts.group, opts.pass_tb); 1: cost-model DP.
 int group_launches(qsv_state *s, const std::vector<PlanItem> &plan, const qsv_run_opts &o,
-                   std::vector<Launch> &launches, double *pred_total);
+                   std::vector<Launch> &launches, double *pred_total, const std::vector<int> *window = nullptr);
 
 }  // namespace qsv

@@ -106,7 +106,7 @@ def test_learned_schedule_does_not_change_results():
 
 
 @pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
-@pytest.mark.parametrize("split", [1, 2])
+@pytest.mark.parametrize("split", [2, 3])
 def test_trained_model_gain_over_greedy(split):
     m = E.CostModel.load()
     rows = []
@@ -126,9 +126,38 @@ def test_trained_model_gain_over_greedy(split):
             f"gain {np.median(tg) / np.median(tl):.3f}x, fidelity {fid:.12f}")
     # the learned schedule groups blocks into shared sweeps where the model
     # predicts a gain; greedy k = 4 sweeps once per block (held-out circuits:
-    # 1.4-1.8x when the model was trained)
+    # 2.1-4.6x when the model was trained)
     gains = [tg / tl for _, tg, tl, _ in rows]
     for (name, tg, tl, fid), g in zip(rows, gains):
-        assert g >= 1.15, name
+        assert g >= 1.5, name
         assert fid >= 1 - 1e-6
-    assert np.exp(np.mean(np.log(gains))) >= 1.3
+    assert np.exp(np.mean(np.log(gains))) >= 2.0
+
+
+@pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
+def test_trained_model_against_the_heuristic_schedulers():
+    """The learned scheduler chooses between block-first plans (k = 2..5) and
+    window plans (k = 4, 5) and how to launch them; on a held-out 28-qubit
+    random circuit it must beat greedy maximal grouping of k = 4 blocks (by
+    ~6% on this depth-24 circuit when the model was trained; ~1.4x at depth
+    200) and stay within 5% of the window planner's one-sweep-per-window
+    schedule."""
+    m = E.CostModel.load()
+    circ = C.random_layers(28, 24, 5151, haar_frac=0.1)
+    methods = (("passes", dict(scheduler=2, group=8)), ("window", dict(scheduler=3)),
+               ("learned", dict(scheduler=1)))
+    ms = {name: [] for name, _ in methods}
+    with E.StateVector(28, E.C64) as sv:
+        sv.attach_costmodel(m)
+        for _ in range(3):                   # round-robin: device drift hits every method alike
+            for name, kw in methods:
+                sv.reset()
+                st = sv.run(circ, fuse_k=4, split=3, **kw)
+                ms[name].append(st.device_ms)
+                if name == "learned":
+                    planner = st.planner
+    t = {name: float(np.median(v)) for name, v in ms.items()}
+    say("28q random: passes %.1f ms, window %.1f ms, learned %.1f ms (planner %d)"
+        % (t["passes"], t["window"], t["learned"], planner))
+    assert t["learned"] < t["passes"]
+    assert t["learned"] <= t["window"] * 1.05
