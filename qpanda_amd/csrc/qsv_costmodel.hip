@@ -457,8 +457,14 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
         for (size_t i = 0; i < c.plan.size(); i++) {
             shape[i].barrier = c.plan[i].barrier;
             shape[i].qubits = c.plan[i].qubits;
-            if (c.plan[i].barrier < 0 && s->n >= std::max(4, (int)c.plan[i].qubits.size()) + 6)
-                shape[i].qubits = widened_qubits(c.plan[i].qubits, std::max(4, (int)c.plan[i].qubits.size()));
+            if (c.plan[i].barrier < 0 && s->n >= std::max(4, (int)c.plan[i].qubits.size()) + 6) {
+                if (c.window) {
+                    const std::vector<int> add = window_widening(c.plan, c.win, i, 4);
+                    shape[i].qubits.insert(shape[i].qubits.end(), add.begin(), add.end());
+                } else {
+                    shape[i].qubits = widened_qubits(c.plan[i].qubits, std::max(4, (int)c.plan[i].qubits.size()));
+                }
+            }
         }
         std::vector<Launch> launches;
         double total = 0;

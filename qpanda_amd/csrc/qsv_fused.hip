@@ -72,6 +72,7 @@ struct FusedArgs {
     int erow[32];            // compact tile position of row (amplitude) r
     int col_rank[8];         // rank in L of the i-th free tile (column) bit
     float bscale, binv;      // FP16x3: state scale 2^s and 2^-s
+    float one;               // 1.0f (see fp16x3_product)
 };
 
 // acc[h] += M B over one K-block pair of bf16 pieces.  ap[p][h][sb]: piece p
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     if constexpr (X6) {
         bf16x6_product<HALVES, LOBLK>(acc, ap, bv);
     } else if constexpr (H3) {
-        fp16x3_product<HALVES, LOBLK>(acc, ah, bv, 1.0f);
+        fp16x3_product<HALVES, LOBLK>(acc, ah, bv, fa.one);
     } else {
 #pragma unroll
         for (int s = 0; s < KSTEPS; s++)
@@ -372,6 +373,7 @@ int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops,
     FusedArgs fa{};
     fa.bscale = s->f16_scale;
     fa.binv = 1.0f / s->f16_scale;
+    fa.one = 1.0f;
     uint64_t qmask = 0;
     for (int j = 0; j < kq; j++) qmask |= uint64_t(1) << qubits[j];
     // the tc lowest free bits are the tile's columns

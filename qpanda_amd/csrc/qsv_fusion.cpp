@@ -304,13 +304,22 @@ int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
 // use (U (x) I, the new qubits taking the high row bits).
 void widen_to(PlanItem &it, int kq)
 {
-    const int k = (int)it.qubits.size();
-    if (k >= kq) return;
     uint64_t used = 0;
     for (int q : it.qubits) used |= uint64_t(1) << q;
+    std::vector<int> add;
+    for (int b = 0; (int)(it.qubits.size() + add.size()) < kq; b++)
+        if (!(used >> b & 1)) add.push_back(b);
+    widen_with(it, add);
+}
+
+// Widen a block with identity on the qubits `add` (not in the block).
+void widen_with(PlanItem &it, const std::vector<int> &add)
+{
+    const int k = (int)it.qubits.size();
+    const int kq = k + (int)add.size();
+    if (add.empty()) return;
     std::vector<int> qs = it.qubits;
-    for (int b = 0; (int)qs.size() < kq; b++)
-        if (!(used >> b & 1)) qs.push_back(b);
+    qs.insert(qs.end(), add.begin(), add.end());
     const int d = 1 << k, D = 1 << kq;
     std::vector<double> w(2 * (size_t)D * D, 0.0);
     for (int hi = 0; hi < D / d; hi++)
@@ -351,7 +360,12 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
     }
     if (rc) return rc;
     const int kq_min = 4;
-    // widen to >= 4 qubits where an MFMA tile fits; other blocks use the FP64 kernel
+    // window plans widen their small blocks with the window's highest qubits
+    // (window_widening); the rest widen to >= 4 qubits with the lowest free
+    // ones where an MFMA tile fits; other blocks use the FP64 kernel
+    if (!window.empty() && s->n >= kq_min + 6)
+        for (size_t i = 0; i < plan.size(); i++)
+            if (plan[i].barrier < 0) widen_with(plan[i], window_widening(plan, window, i, kq_min));
     std::vector<char> mfma(plan.size(), 0);
     for (size_t i = 0; i < plan.size(); i++) {
         PlanItem &it = plan[i];

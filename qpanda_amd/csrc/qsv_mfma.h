@@ -10,7 +10,7 @@
 //   * the three products a1 b0 + a0 b1 + a0 b0 (smallest first) accumulate in
 //     FP32 and the result is multiplied by 1/s (exact).
 // The kernels keep a tile scaled by s in LDS from its load to its store, so
-// the blocks in between call fp16x3_product with scale 1 (no multiply).
+// the blocks in between split it without a multiply.
 // The dropped terms (a1 b1, r, r') are ~2^-22 of each product: the operands
 // keep 22 of FP32's 24 significand bits, against BF16x6's 24, for half of its
 // matrix-core work (3 MFMAs per K-block instead of 6).  s = 2^15 for a state
@@ -33,19 +33,24 @@ typedef _Float16 f16x2v __attribute__((ext_vector_type(2)));
 // acc[h] += M B over LOBLK K-blocks.  ap[p][h][sb]: fp16 piece p of the A
 // operand in the permuted K order of the BF16 products (K-block sb, lane half
 // hf, element e <-> real column 2 (8 sb + e) + hf), so the B values of K-block
-// sb are this lane's FP32 B values bv[8 sb .. 8 sb + 7].
+// sb are this lane's FP32 B values bv[8 sb .. 8 sb + 7], already scaled.
+// `one` must be 1.0f but unknown to the compiler (a kernel argument): the
+// residual x * one - fp16(x) then compiles to one v_fma_mix_f32 that reads the
+// fp16 half in place, where x - fp16(x) with a literal costs a conversion
+// and a subtraction (16 VALU instructions more per 32-column chunk).
 template <int HALVES, int LOBLK>
 __device__ __forceinline__ void fp16x3_product(floatx16 (&acc)[HALVES], const f16x8 (&ap)[2][HALVES][LOBLK],
-                                               const float *bv, float scale)
+                                               const float *bv, float one)
 {
 #pragma unroll
     for (int sb = 0; sb < LOBLK; sb++) {
         f16x8 b0, b1;
 #pragma unroll
         for (int e = 0; e < 8; e += 2) {
-            const float x0 = bv[8 * sb + e] * scale, x1 = bv[8 * sb + e + 1] * scale;
+            const float x0 = bv[8 * sb + e], x1 = bv[8 * sb + e + 1];
             const f16x2v h = __builtin_convertvector((f32x2v){x0, x1}, f16x2v);
-            const f16x2v l = __builtin_convertvector((f32x2v){x0 - (float)h[0], x1 - (float)h[1]}, f16x2v);
+            const f16x2v l = __builtin_convertvector(
+                (f32x2v){__builtin_fmaf(x0, one, -(float)h[0]), __builtin_fmaf(x1, one, -(float)h[1])}, f16x2v);
             b0[e] = h[0];
             b0[e + 1] = h[1];
             b1[e] = l[0];

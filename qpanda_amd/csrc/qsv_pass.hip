@@ -71,6 +71,7 @@ struct PassArgs {
     uint64_t hioff[16];     // pdep(512 i, L): address part of load i (i < 2^(TB-9))
     int nblocks;
     float bscale, binv;     // FP16x3: state scale 2^s and 2^-s
+    float one;              // 1.0f (see fp16x3_product)
     PassBlock b[kMaxPassBlocks];
 };
 
@@ -374,15 +375,15 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     __syncthreads();
     if (ANY5) {
         for (int g = 0; g < pa.nblocks; g++) {
-            if (pa.b[g].kq == 4) pass_apply<4, MODE, NW>(pre, pim, pa.b[g], lane, wave, 1.0f, 1.0f);
-            else pass_apply<5, MODE, NW>(pre, pim, pa.b[g], lane, wave, 1.0f, 1.0f);
+            if (pa.b[g].kq == 4) pass_apply<4, MODE, NW>(pre, pim, pa.b[g], lane, wave, pa.one, 1.0f);
+            else pass_apply<5, MODE, NW>(pre, pim, pa.b[g], lane, wave, pa.one, 1.0f);
             __syncthreads();
         }
     } else {
         for (int g = 0; g < pa.nblocks; g++) {
             Ops4<MODE> ops;
             load_ops4<MODE>(ops, pa.b[g].mops, lane);
-            pass_apply4<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, 1.0f, 1.0f);
+            pass_apply4<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, pa.one, 1.0f);
             __syncthreads();
         }
     }
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(kPipeThreads) void k_fused_pass_pipe(float2 *__rest
             for (int g = 0; g < G; g++) {
                 Ops4<MODE> ops;
                 load_ops4<MODE>(ops, pa.b[g].mops, lane);
-                pass_apply4<TB, MODE, kPipeCompute>(pre, pa.b[g], ops, lane, wave, 1.0f, 1.0f);
+                pass_apply4<TB, MODE, kPipeCompute>(pre, pa.b[g], ops, lane, wave, pa.one, 1.0f);
                 __syncthreads();
             }
         }
@@ -592,6 +593,7 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     PassArgs pa{};
     pa.bscale = s->f16_scale;
     pa.binv = 1.0f / s->f16_scale;
+    pa.one = 1.0f;
     pa.tile_ins = insert_set_from_mask(lmask);
     const int *lpos = pa.tile_ins.pos;
     for (int j = 0; j < tb; j++) pa.lpos[j] = lpos[j];

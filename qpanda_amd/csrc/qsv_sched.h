@@ -29,12 +29,18 @@ struct SchedChoice {
 int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
                 std::vector<PlanItem> &plan);
 void widen_to(PlanItem &it, int kq);
+void widen_with(PlanItem &it, const std::vector<int> &add);
 
 // Window planner (qsv_window.cpp): blocks of <= max_k (4, 5) qubits grouped
 // into windows of tb tile bits; window[i] numbers the window of plan[i]
 // (barrier records get their own).
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
                 std::vector<PlanItem> &plan, std::vector<int> &window);
+// Qubits that widen block i of a window plan to kq qubits: the highest of
+// its window's qubits (the union of the window's blocks and qubits 0..3)
+// that it does not use.
+std::vector<int> window_widening(const std::vector<PlanItem> &plan, const std::vector<int> &window, size_t i,
+                                 int kq);
 
 // Learned scheduler (qsv_costmodel.hip): plans at every k, predicts each
 // block's time with the cost model, keeps the cheapest plan.

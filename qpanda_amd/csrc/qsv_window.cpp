@@ -202,6 +202,29 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
     return QSV_OK;
 }
 
+// A block's rows are its qubits' tile bits and its columns the others; the 32
+// lanes of a half-wave take consecutive columns.  Widening with the lowest
+// free qubits (0..3) would make the tile's lowest bits rows, so the lanes'
+// LDS addresses would step by 2^(rows below them) and collide in banks;
+// widening with the window's highest qubits keeps the low tile bits as
+// columns.  Every qubit chosen is already in the window's tile.
+std::vector<int> window_widening(const std::vector<PlanItem> &plan, const std::vector<int> &window, size_t i,
+                                 int kq)
+{
+    std::vector<int> add;
+    const PlanItem &it = plan[i];
+    if (it.barrier >= 0 || (int)it.qubits.size() >= kq) return add;
+    uint64_t u = 0xF;
+    for (size_t j = i; j > 0 && window[j - 1] == window[i]; j--)
+        for (int q : plan[j - 1].qubits) u |= uint64_t(1) << q;
+    for (size_t j = i; j < plan.size() && window[j] == window[i]; j++)
+        for (int q : plan[j].qubits) u |= uint64_t(1) << q;
+    for (int q : it.qubits) u &= ~(uint64_t(1) << q);
+    for (int b = 63; b >= 0 && (int)(it.qubits.size() + add.size()) < kq; b--)
+        if (u >> b & 1) add.push_back(b);
+    return add;
+}
+
 }  // namespace qsv
 
 using namespace qsv;
