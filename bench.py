@@ -427,7 +427,8 @@ def main_large(a):
         norm = sv.norm2()
         result["norm2_fp64"] = norm
         cl = st.classes()
-        result["fused"] = {"k": int(st.fuse_k), "product": a.product, "scheduler": "learned", "blocks": int(st.blocks),
+        result["fused"] = {"k": int(st.fuse_k), "product": a.product, "scheduler": "learned",
+                           "planner": "window" if st.planner else "block-first", "blocks": int(st.blocks),
                            "sweeps": int(sum(cl.get(c, {}).get("launches", 0) for c in ("fused", "pass", "block64"))),
                            "plan_ms": st.plan_ms, "device_ms": st.device_ms, "predicted_ms": st.model_ms_pred}
         # shots: all qubits in order from the final state, one draw per gate skipped per shot

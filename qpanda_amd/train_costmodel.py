@@ -5,7 +5,8 @@
                                          [--evaluate PARAMS]
 
 1. Collect: seeded circuits (random layers, QFT, hardware-efficient ansatz) at
-   several qubit counts run on the fused path under every decision the
+   qubit counts 20..30 (and 32, with the products and shapes that matter
+   there) run on the fused path under every decision the
    scheduler can make -- block size k = 2..5, the four products (FP32, FP32
    + BF16 lo correction, BF16x6, FP16x3), single-block launches at every tile
    shape (6, 7, 8 column bits), multi-block sweeps of up to 2, 3, 4, 8 or 16
@@ -38,6 +39,12 @@ SINGLE = [dict(fuse_k=k, split=s, tile=t, scheduler=0) for k in (2, 3, 4, 5) for
 PASSES = [dict(fuse_k=k, split=s, scheduler=2, group=g, pass_tb=tb) for k in (3, 4, 5) for s in (1, 2, 3)
           for g in (2, 3, 4, 8, 16) for tb in (0, 12, 13)]
 WINDOWS = [dict(fuse_k=k, split=s, scheduler=3, pass_tb=tb) for k in (4, 5) for s in (1, 2, 3) for tb in (12, 13)]
+# above 30 qubits (32 GiB states) only the decisions that matter at that size:
+# the FP16x3 / BF16x6 products, k = 4 and 5, full tiles
+LARGE = ([dict(fuse_k=k, split=s, tile=7, scheduler=0) for k in (4, 5) for s in (2, 3)]
+         + [dict(fuse_k=k, split=s, scheduler=2, group=g, pass_tb=tb) for k in (4, 5) for s in (2, 3)
+            for g in (4, 8, 16) for tb in (0, 13)]
+         + [dict(fuse_k=k, split=s, scheduler=3, pass_tb=13) for k in (4, 5) for s in (2, 3)])
 
 
 def workloads(ns, seed0, depth):
@@ -59,7 +66,7 @@ def collect(ns, seed0=7, depth=6, log=print):
     for n, circs in sorted(by_n.items()):
         with E.StateVector(n, E.C64) as sv:
             for circ in circs:
-                for cfg in SINGLE + PASSES + WINDOWS:
+                for cfg in (SINGLE + PASSES + WINDOWS) if n <= 30 else LARGE:
                     sv.reset()
                     sv.run(circ, **cfg)                           # warm
                     sv.reset()
@@ -68,6 +75,7 @@ def collect(ns, seed0=7, depth=6, log=print):
                     keep = ms > 0
                     feats.append(f[keep])
                     targ.append(np.log(ms[keep].astype(np.float64) * 2.0 ** (30 - n)))
+                log(f"  {circ.name}: {sum(x.shape[0] for x in feats)} records so far")
         log(f"collected n={n}: {sum(x.shape[0] for x in feats)} records so far")
     return np.concatenate(feats).astype(np.float32), np.concatenate(targ).astype(np.float32)
 
@@ -178,7 +186,7 @@ def main(argv=None):
         return
 
     t0 = time.time()
-    ns = (20, 24, 28) if a.quick else (20, 22, 24, 26, 28, 30)
+    ns = (20, 24, 28) if a.quick else (20, 22, 24, 26, 28, 30, 32)
     x, t = collect(ns, log=log)
     g = np.random.default_rng(0)
     perm = g.permutation(x.shape[0])
