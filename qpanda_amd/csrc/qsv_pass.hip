@@ -4,16 +4,17 @@
 // (the blocks' qubits) u {lowest other qubits until |L| = TB}, TB = 11..13
 // (2^TB complex64 amplitudes = 16..64 KiB of LDS, so several workgroups share
 // a CU); a tile is the 2^TB amplitudes whose non-L bits are fixed, made of
-// contiguous runs of >= 16 amplitudes (128 B).  One workgroup per tile:
+// contiguous runs of >= 16 amplitudes (128 B).  One workgroup of NT threads
+// (pass_threads) per tile:
 //   load   16-byte loads issued first; element e of the tile sits at
-//          pdep(e, L), split as pdep(2 tid, L) + pdep(512 i, L) so each thread
-//          computes its address part once;
+//          pdep(e, L), split as pdep(2 tid, L) + pdep(2 NT i, L) so each
+//          thread computes its address part once;
 //   apply  for each block in order: its 2^(TB-k) groups are the columns; each
 //          wave takes 32-column chunks and runs the same MFMA product as the
-//          single-block kernel (FP32 v_mfma_f32_32x32x2f32 for M_hi, BF16
-//          v_mfma_f32_32x32x16_bf16 for the M_lo correction; or the BF16x6
-//          product on the BF16 MFMA alone), writing the results back in
-//          place; a barrier separates blocks;
+//          single-block kernel (FP16x3 on the FP16 MFMA, BF16x6 on the BF16
+//          MFMA, or FP32 v_mfma_f32_32x32x2f32 with an optional BF16 lo
+//          correction), writing the results back in place; a barrier
+//          separates blocks;
 //   store  16-byte stores to the same addresses.
 // LDS layout: tile element e sits at float index P(e) = e ^ S(e) of its
 // plane, S XOR-ing bank bits 1..4 with a fixed linear function of bits 5..8
@@ -58,7 +59,7 @@ __host__ __device__ inline int lds_swizzle(int e)
 }
 
 struct PassBlock {
-    const float *mops;      // operand-ready matrix (hi floats, then bf16 lo)
+    const float *mops;      // operand-ready matrix (layout of fused_operands for the product)
     int kq;                 // 4 or 5
     int ncolbits;           // TB - kq
     int erow[32];           // swizzled LDS index of row (amplitude) r
@@ -68,7 +69,7 @@ struct PassBlock {
 struct PassArgs {
     InsertSet tile_ins;     // L positions, ascending
     int lpos[16];           // compact tile bit j -> qubit
-    uint64_t hioff[16];     // pdep(512 i, L): address part of load i (i < 2^(TB-9))
+    uint64_t hioff[16];     // pdep(2 NT i, L): address part of load i (NT = threads that move the tile)
     int nblocks;
     float bscale, binv;     // FP16x3: state scale 2^s and 2^-s
     float one;              // 1.0f (see fp16x3_product)
