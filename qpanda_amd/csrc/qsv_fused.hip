@@ -185,11 +185,13 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     // FP16x3: the tile sits in LDS scaled by 2^s (exact), so the split needs
     // no multiply; the store scales back
     const float ls = H3 ? fa.bscale : 1.0f, li = H3 ? fa.binv : 1.0f;
+    // (e, e + 1) as one aligned 8-byte access per plane (two dword accesses
+    // at stride 2 would collide 2-way in the banks)
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
-        pre[e] = buf[i].x * ls; pim[e] = buf[i].y * ls;
-        pre[e + 1] = buf[i].z * ls; pim[e + 1] = buf[i].w * ls;
+        *reinterpret_cast<float2 *>(pre + e) = make_float2(buf[i].x * ls, buf[i].z * ls);
+        *reinterpret_cast<float2 *>(pim + e) = make_float2(buf[i].y * ls, buf[i].w * ls);
     }
     __syncthreads();
 
@@ -246,9 +248,10 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
+        const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
         float4 v;
-        v.x = pre[e] * li; v.y = pim[e] * li;
-        v.z = pre[e + 1] * li; v.w = pim[e + 1] * li;
+        v.x = r.x * li; v.y = m.x * li;
+        v.z = r.y * li; v.w = m.y * li;
         *reinterpret_cast<float4 *>(a + off[i]) = v;
     }
 }

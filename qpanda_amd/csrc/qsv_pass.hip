@@ -367,11 +367,13 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     // FP16x3: the tile sits in LDS scaled by 2^s (exact) from staging to the
     // store, so the blocks split it without a multiply on either side
     const float ls = MODE == 3 ? pa.bscale : 1.0f, li = MODE == 3 ? pa.binv : 1.0f;
+    // (e, e + 1) move as one aligned 8-byte access per plane: as two dword
+    // accesses a half-wave's even addresses would collide 2-way in the banks
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
         const int e = (2 * (tid + NT * i)) ^ sw;
-        pre[e] = buf[i].x * ls; pim[e] = buf[i].y * ls;
-        pre[e + 1] = buf[i].z * ls; pim[e + 1] = buf[i].w * ls;
+        *reinterpret_cast<float2 *>(pre + e) = make_float2(buf[i].x * ls, buf[i].z * ls);
+        *reinterpret_cast<float2 *>(pim + e) = make_float2(buf[i].y * ls, buf[i].w * ls);
     }
     __syncthreads();
     if (ANY5) {
@@ -391,9 +393,10 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
         const int e = (2 * (tid + NT * i)) ^ sw;
+        const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
         float4 v;
-        v.x = pre[e] * li; v.y = pim[e] * li;
-        v.z = pre[e + 1] * li; v.w = pim[e + 1] * li;
+        v.x = r.x * li; v.y = m.x * li;
+        v.z = r.y * li; v.w = m.y * li;
         *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
     }
 }
@@ -465,8 +468,8 @@ __global__ __launch_bounds__(kPipeThreads) void k_fused_pass_pipe(float2 *__rest
 #pragma unroll
             for (int j = 0; j < NLD; j++) {
                 const int e = (2 * (mt + kPipeMemThreads * j)) ^ sw;
-                pre[e] = buf[j].x * ls; pim[e] = buf[j].y * ls;
-                pre[e + 1] = buf[j].z * ls; pim[e + 1] = buf[j].w * ls;
+                *reinterpret_cast<float2 *>(pre + e) = make_float2(buf[j].x * ls, buf[j].z * ls);
+                *reinterpret_cast<float2 *>(pim + e) = make_float2(buf[j].y * ls, buf[j].w * ls);
             }
         };
         auto drain = [&](int bi, uint64_t i) {
@@ -475,9 +478,10 @@ __global__ __launch_bounds__(kPipeThreads) void k_fused_pass_pipe(float2 *__rest
 #pragma unroll
             for (int j = 0; j < NLD; j++) {
                 const int e = (2 * (mt + kPipeMemThreads * j)) ^ sw;
+                const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
                 float4 v;
-                v.x = pre[e] * li; v.y = pim[e] * li;
-                v.z = pre[e + 1] * li; v.w = pim[e + 1] * li;
+                v.x = r.x * li; v.y = m.x * li;
+                v.z = r.y * li; v.w = m.y * li;
                 *reinterpret_cast<float4 *>(a + b + pa.hioff[j]) = v;
             }
         };
