@@ -3,7 +3,7 @@
 Each instance's sum of <Z_q Z_{q+1}> must match the oracle's value for the same
 gate list (tests restate the instance with qpanda_amd.engine.hea_circuit) within
 the complex64 state's rounding (absolute 2e-4 on a sum of up to 21 terms).
-Instances of 10..14 qubits run inside LDS, 15..17 through the two-pass HBM
+Instances of up to 14 qubits run inside LDS, 15..22 through the two-pass HBM
 path.  hipGraph replays must reproduce the eager results exactly, replay after
 replay."""
 import sys
@@ -81,3 +81,19 @@ def test_full_configuration_4096_instances():
         if idx:
             i = idx[0]
             assert abs(zz[i] - oracle_zz(n, ang[i])) < 2e-4
+
+
+def test_two_pass_path_at_the_largest_sizes():
+    """The second pass's tile holds qubits 13..n-1 plus the 27 - n lowest (5 of
+    them at 22 qubits).  Every size of the HBM path, and the LDS path's
+    largest, against the FP64 oracle at every thread count."""
+    nq = [15, 16, 18, 19, 20, 21, 22, 14]
+    ly = [3, 2, 2, 1, 2, 1, 2, 2]
+    g = np.random.default_rng(31)
+    ang = [g.uniform(-np.pi, np.pi, (l, n, 2)) for n, l in zip(nq, ly)]
+    b = E.Batch(nq, ly, ang)
+    for t in (256, 512, 1024):
+        zz, _, _ = b.run(t)
+        say("threads", t, "sizes", nq)
+        for n, a, z in zip(nq, ang, zz):
+            assert abs(z - oracle_zz(n, a)) < 2e-4, (t, n)
