@@ -11,6 +11,7 @@ against the FP64 oracle state.  Block qubit sets cover the low run of qubits
 tile-addressing case of the kernel is exercised.
 """
 import math
+import os
 import sys
 
 import numpy as np
@@ -271,3 +272,26 @@ def test_pipelined_sweep_matches_tile_sweep(n, split):
         assert st.classes().get("pass", {}).get("launches", 0) > 0
     say("pipelined vs tile sweep n", n, "max diff", np.abs(outs[0] - outs[1]).max())
     assert np.abs(outs[0] - outs[1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("n", [8, 10, 11, 12, 13, 14])
+def test_window_and_learned_schedulers_at_small_sizes(n):
+    """Edges of the window planner: below 11 qubits scheduler 3 falls back to
+    one sweep per block; at 11..13 one window covers the whole state and its
+    blocks are split into launches of <= 16; the learned scheduler picks among
+    its candidates at every size.  States against the FP64 oracle, draws
+    counted as the reference does."""
+    circ = C.random_layers(n, 24, 6100 + n, haar_frac=0.1)
+    ref, _ = O.run_circuit(circ)
+    m = E.CostModel.load() if os.path.exists(E.DEFAULT_COSTMODEL) else E.CostModel(5)
+    for sched in (3, 1):
+        with E.StateVector(n, E.C64) as sv:
+            if sched == 1:
+                sv.attach_costmodel(m)
+            st = sv.run(circ, fuse_k=4, split=3, scheduler=sched, rng_state=77)
+            got = sv.get()
+            fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
+        say("n", n, "scheduler", sched, "planner", st.planner, "launches", st.launches, "fidelity", fid)
+        assert (st.planner == 1) == (sched == 3 and n >= 11) or sched == 1
+        assert fid >= 1 - 1e-6
+        assert np.abs(got - ref.psi).max() < 1e-5
