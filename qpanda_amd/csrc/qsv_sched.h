@@ -30,6 +30,9 @@ int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
                 std::vector<PlanItem> &plan);
 void widen_to(PlanItem &it, int kq);
 void widen_with(PlanItem &it, const std::vector<int> &add);
+// Merge narrow blocks into commuting neighbours while the union stays within
+// max_k qubits (the plan's product is unchanged).
+void absorb_blocks(std::vector<PlanItem> &plan, int max_k);
 
 // Window planner (qsv_window.cpp): blocks of <= max_k (4, 5) qubits grouped
 // into windows of tb tile bits; window[i] numbers the window of plan[i]

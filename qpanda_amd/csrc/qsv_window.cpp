@@ -14,7 +14,8 @@
 //      qubit, until the window holds TB qubits or nothing gains;
 //   2. take the runnable gates in dependency order (front-of-queue closure)
 //      and fuse them into blocks of <= max_k qubits with plan_greedy, so every
-//      block lies inside the window;
+//      block lies inside the window (plan_greedy ends by merging narrow blocks
+//      into commuting neighbours, absorb_blocks);
 //   3. emit the blocks as one window.
 // On 30-qubit random circuits of depth 200 (TB = 13) this applies ~31 gates
 // per sweep against ~21 for block-first grouping, with more (but cheaper per

@@ -169,3 +169,30 @@ def test_window_plan_needs_fewer_sweeps_than_block_first_grouping():
             sweeps += 1
             cur = set(hi)
     assert windows <= 300 < 400 < sweeps, (windows, sweeps)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+def test_plans_leave_no_mergeable_neighbours(k):
+    """Neighbouring blocks commute into one another's place trivially, so the
+    planners merge every neighbouring pair whose union has <= k qubits: none
+    may be left (block-first plans, and within each window of a window
+    plan)."""
+    circ = C.random_layers(16, 30, 77 + k, haar_frac=0.1)
+    plan = E.fuse_plan(circ, k)
+    for (qa, _, _), (qb, _, _) in zip(plan, plan[1:]):
+        assert len(set(qa) | set(qb)) > k, (qa, qb)
+    if k >= 4:
+        wplan = E.window_plan(circ, k, 10)
+        for (qa, _, _, wa), (qb, _, _, wb) in zip(wplan, wplan[1:]):
+            if wa == wb:
+                assert len(set(qa) | set(qb)) > k, (qa, qb)
+
+
+def test_window_blocks_are_mostly_full_width():
+    """The benchmark circuit (30 qubits, depth 200, k = 4): merging narrow
+    blocks into their neighbours leaves no single-qubit block and at least
+    60% of the blocks at the full 4 qubits (greedy fusion alone: 41%)."""
+    circ = C.random_layers(30, 200, 2025, haar_frac=0.1)
+    widths = [len(qs) for qs, *_ in E.window_plan(circ, 4, 13) if qs is not None]
+    assert min(widths) >= 2
+    assert sum(w == 4 for w in widths) >= 0.6 * len(widths), np.bincount(widths)
