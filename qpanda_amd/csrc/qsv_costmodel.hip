@@ -419,7 +419,10 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
     // candidate plans: block-first at k = 2..5 and window plans at k = 4, 5
     // (states of >= 11 qubits).  Planning is host work with no shared state,
     // so the candidates are built on threads of their own; the cost-model DP
-    // (device inference on the model's stream) then scores them in turn.
+    // (device inference on the model's stream) then scores them in turn.  The
+    // candidates are planned as shapes (qubit lists, no block matrices: the
+    // planners' decisions depend on the qubits alone) and only the winner is
+    // planned again with its matrices.
     struct Cand {
         int k = 0, window = 0, rc = QSV_OK;
         std::string err;                 // the planner's message (errors are per thread)
@@ -442,8 +445,8 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
     std::vector<std::thread> pool;
     for (Cand &c : cands)
         pool.emplace_back([&c, gates, ng, mats, &o, s]() {
-            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win)
-                            : plan_greedy(gates, ng, mats, c.k, o.exact, c.plan);
+            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win, true)
+                            : plan_greedy(gates, ng, mats, c.k, o.exact, c.plan, true);
             if (c.rc) c.err = qsv_last_error();
         });
     for (std::thread &t : pool) t.join();
@@ -476,8 +479,13 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
         }
     }
     if (!best) return fail(QSV_ERR_NUMERIC, "cost model predicted no finite schedule");
-    plan.swap(best->plan);
-    window->swap(best->win);
+    int rc = best->window ? plan_window(gates, ng, mats, best->k, o.exact, kMaxPassTileBits, s->n, plan, *window)
+                          : plan_greedy(gates, ng, mats, best->k, o.exact, plan);
+    if (rc) return rc;
+    bool same = plan.size() == best->plan.size() && window->size() == best->win.size();
+    for (size_t i = 0; same && i < plan.size(); i++)
+        same = plan[i].qubits == best->plan[i].qubits && plan[i].barrier == best->plan[i].barrier;
+    if (!same) return fail(QSV_ERR_UNSUPPORTED, "planner shapes differ with and without block matrices");
     choice->fuse_k = best->k;
     choice->window = best->window;
     choice->pred_ms = best_total;

@@ -123,29 +123,43 @@ int gate_matrix(const qsv_gate &r, const double *mats, int exact, int *qs, std::
 
 // Apply an m-qubit matrix g (on block positions pos[0..m-1]) to every column
 // of the block matrix B (dimension 2^nq): B <- G B.
+// Blocks have at most 5 qubits (D <= 32).  The rows of B are combined whole,
+// with the complex products written out in real arithmetic: std::complex's
+// operator* goes through the C99 Annex G helper (__muldc3), which made this
+// loop most of the planning time of k = 5 plans.
 void apply_small(std::vector<cd> &B, int nq, const int *pos, int m, const std::vector<cd> &g)
 {
     const int D = 1 << nq, d = 1 << m;
     int mask = 0;
     for (int j = 0; j < m; j++) mask |= 1 << pos[j];
-    std::vector<int> off(d);
+    int off[32];
     for (int r = 0; r < d; r++) {
         int o = 0;
         for (int j = 0; j < m; j++)
             if (r >> j & 1) o |= 1 << pos[j];
         off[r] = o;
     }
-    std::vector<cd> x(d);
-    for (int col = 0; col < D; col++)
-        for (int base = 0; base < D; base++) {
-            if (base & mask) continue;
-            for (int c = 0; c < d; c++) x[c] = B[(size_t)(base | off[c]) * D + col];
-            for (int r = 0; r < d; r++) {
-                cd acc = 0;
-                for (int c = 0; c < d; c++) acc += g[(size_t)r * d + c] * x[c];
-                B[(size_t)(base | off[r]) * D + col] = acc;
+    double *b = reinterpret_cast<double *>(B.data());
+    const double *gm = reinterpret_cast<const double *>(g.data());
+    double acc[2 * 32 * 32];
+    for (int base = 0; base < D; base++) {
+        if (base & mask) continue;
+        for (int r = 0; r < d; r++) {
+            double *a = acc + 2 * r * D;
+            for (int col = 0; col < 2 * D; col++) a[col] = 0;
+            for (int c = 0; c < d; c++) {
+                const double gr = gm[2 * (r * d + c)], gi = gm[2 * (r * d + c) + 1];
+                if (gr == 0 && gi == 0) continue;
+                const double *x = b + 2 * (size_t)(base | off[c]) * D;
+                for (int col = 0; col < D; col++) {
+                    a[2 * col] += gr * x[2 * col] - gi * x[2 * col + 1];
+                    a[2 * col + 1] += gr * x[2 * col + 1] + gi * x[2 * col];
+                }
             }
         }
+        for (int r = 0; r < d; r++)
+            std::memcpy(b + 2 * (size_t)(base | off[r]) * D, acc + 2 * r * D, sizeof(double) * 2 * D);
+    }
 }
 
 struct Block {
@@ -184,7 +198,7 @@ void widen(Block &b, const std::vector<int> &qs)
 }  // namespace
 
 int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
-                std::vector<PlanItem> &plan)
+                std::vector<PlanItem> &plan, bool shape_only)
 {
     if (max_k < 1 || max_k > 5) return fail(QSV_ERR_ARG, "fusion size must be 1..5");
     std::vector<Block> blocks;
@@ -260,19 +274,20 @@ int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
                 if (index_of(u, q) < 0) u.push_back(q);
         for (int j = 0; j < nq; j++)
             if (index_of(u, qs[j]) < 0) u.push_back(qs[j]);
-        widen(blocks[ti], u);
+        if (shape_only) blocks[ti].qs = u;
+        else widen(blocks[ti], u);
         for (size_t c = 1; c < keep.size(); c++) {
             Block &o = blocks[keep[c]];
             std::vector<int> pos(o.qs.size());
             for (size_t j = 0; j < o.qs.size(); j++) pos[j] = index_of(u, o.qs[j]);
-            apply_small(blocks[ti].m, (int)u.size(), pos.data(), (int)o.qs.size(), o.m);
+            if (!shape_only) apply_small(blocks[ti].m, (int)u.size(), pos.data(), (int)o.qs.size(), o.m);
             blocks[ti].ngates += o.ngates;
             o.alive = false;
             o.open = false;
         }
         int pos[3];
         for (int j = 0; j < nq; j++) pos[j] = index_of(u, qs[j]);
-        apply_small(blocks[ti].m, (int)u.size(), pos, nq, gm);
+        if (!shape_only) apply_small(blocks[ti].m, (int)u.size(), pos, nq, gm);
         Block &t = blocks[ti];
         t.ngates += 1;
         t.seq = ++clock;
@@ -297,7 +312,7 @@ int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
         }
         plan.push_back(std::move(it));
     }
-    absorb_blocks(plan, max_k);
+    absorb_blocks(plan, max_k, shape_only);
     return QSV_OK;
 }
 
@@ -351,12 +366,18 @@ std::vector<cd> complex_of(const std::vector<double> &u)
 }
 
 // The block "first, then second" on the union of their qubits (second's
-// qubits first, then first's others).
-PlanItem product_block(const PlanItem &first, const PlanItem &second)
+// qubits first, then first's others); only the qubits if shape_only.
+PlanItem product_block(const PlanItem &first, const PlanItem &second, bool shape_only)
 {
     std::vector<int> u = second.qubits;
     for (int q : first.qubits)
         if (index_of(u, q) < 0) u.push_back(q);
+    if (shape_only) {
+        PlanItem out;
+        out.qubits = u;
+        out.ngates = first.ngates + second.ngates;
+        return out;
+    }
     Block b;
     b.qs = first.qubits;
     b.m = complex_of(first.u);
@@ -385,7 +406,7 @@ PlanItem product_block(const PlanItem &first, const PlanItem &second)
 // meets one that shares a qubit; merge it into the reachable block (the one it
 // meets included) whose union with it is narrowest, if that is <= max_k
 // qubits.  Barrier records are never crossed.
-void absorb_blocks(std::vector<PlanItem> &plan, int max_k)
+void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only)
 {
     bool changed = true;
     while (changed) {
@@ -414,8 +435,8 @@ void absorb_blocks(std::vector<PlanItem> &plan, int max_k)
                 if (mc & mb) break;
             }
             if (best == b) continue;
-            if (best > b) plan[best] = product_block(plan[b], plan[best]);
-            else plan[best] = product_block(plan[best], plan[b]);
+            if (best > b) plan[best] = product_block(plan[b], plan[best], shape_only);
+            else plan[best] = product_block(plan[best], plan[b], shape_only);
             plan.erase(plan.begin() + (std::ptrdiff_t)b);
             b--;                    // the next block moved into slot b
             changed = true;

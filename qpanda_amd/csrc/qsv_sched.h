@@ -26,19 +26,21 @@ struct SchedChoice {
     double pred_ms = 0;
 };
 
+// shape_only: qubit lists and gate counts only (PlanItem::u left empty); the
+// fusion decisions depend on the qubits alone, so the blocks are the same.
 int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
-                std::vector<PlanItem> &plan);
+                std::vector<PlanItem> &plan, bool shape_only = false);
 void widen_to(PlanItem &it, int kq);
 void widen_with(PlanItem &it, const std::vector<int> &add);
 // Merge narrow blocks into commuting neighbours while the union stays within
 // max_k qubits (the plan's product is unchanged).
-void absorb_blocks(std::vector<PlanItem> &plan, int max_k);
+void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only = false);
 
 // Window planner (qsv_window.cpp): blocks of <= max_k (4, 5) qubits grouped
 // into windows of tb tile bits; window[i] numbers the window of plan[i]
 // (barrier records get their own).
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
-                std::vector<PlanItem> &plan, std::vector<int> &window);
+                std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only = false);
 // Qubits that widen block i of a window plan to kq qubits: the highest of
 // its window's qubits (the union of the window's blocks and qubits 0..3)
 // that it does not use.

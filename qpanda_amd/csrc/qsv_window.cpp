@@ -96,7 +96,7 @@ struct Segment {
 }  // namespace
 
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
-                std::vector<PlanItem> &plan, std::vector<int> &window)
+                std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only)
 {
     if (max_k < 4 || max_k > 5) return fail(QSV_ERR_ARG, "window planning fuses blocks of 4 or 5 qubits");
     if (n < 4 || n > 62) return fail(QSV_ERR_ARG, "window planning needs 4..62 qubits");
@@ -180,7 +180,7 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
             left -= got;
             sub.clear();
             for (int64_t g : taken) sub.push_back(sg.gates[g]);
-            int rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks);
+            int rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, shape_only);
             if (rc) return rc;
             // no-op records fuse to nothing; keep them counted (one draw each)
             int64_t counted = 0;
