@@ -422,12 +422,14 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
     // (device inference on the model's stream) then scores them in turn.  The
     // candidates are planned as shapes (qubit lists, no block matrices: the
     // planners' decisions depend on the qubits alone) and only the winner is
-    // planned again with its matrices.
+    // planned again with its matrices (a window plan replays its windows'
+    // qubit masks instead of growing them again).
     struct Cand {
         int k = 0, window = 0, rc = QSV_OK;
         std::string err;                 // the planner's message (errors are per thread)
         std::vector<PlanItem> plan;
         std::vector<int> win;
+        std::vector<uint64_t> masks;     // window plans: each window's qubits
     };
     std::vector<Cand> cands;
     for (int k = 2; k <= 5; k++) {
@@ -445,7 +447,8 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
     std::vector<std::thread> pool;
     for (Cand &c : cands)
         pool.emplace_back([&c, gates, ng, mats, &o, s]() {
-            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win, true)
+            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win, true,
+                                          nullptr, &c.masks)
                             : plan_greedy(gates, ng, mats, c.k, o.exact, c.plan, true);
             if (c.rc) c.err = qsv_last_error();
         });
@@ -479,7 +482,8 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
         }
     }
     if (!best) return fail(QSV_ERR_NUMERIC, "cost model predicted no finite schedule");
-    int rc = best->window ? plan_window(gates, ng, mats, best->k, o.exact, kMaxPassTileBits, s->n, plan, *window)
+    int rc = best->window ? plan_window(gates, ng, mats, best->k, o.exact, kMaxPassTileBits, s->n, plan, *window,
+                                        false, &best->masks)
                           : plan_greedy(gates, ng, mats, best->k, o.exact, plan);
     if (rc) return rc;
     bool same = plan.size() == best->plan.size() && window->size() == best->win.size();

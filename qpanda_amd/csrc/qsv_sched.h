@@ -39,8 +39,11 @@ void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only = fal
 // Window planner (qsv_window.cpp): blocks of <= max_k (4, 5) qubits grouped
 // into windows of tb tile bits; window[i] numbers the window of plan[i]
 // (barrier records get their own).
+// masks_out receives each window's qubit mask; masks_in replays such masks
+// (same gates, tb and n) instead of growing the windows again.
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
-                std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only = false);
+                std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only = false,
+                const std::vector<uint64_t> *masks_in = nullptr, std::vector<uint64_t> *masks_out = nullptr);
 // Qubits that widen block i of a window plan to kq qubits: the highest of
 // its window's qubits (the union of the window's blocks and qubits 0..3)
 // that it does not use.

@@ -8,27 +8,35 @@
 // gate whose qubits all lie in that set can run in the sweep once every
 // earlier gate on its qubits has run.  Per segment between MEASURE / RESET
 // records, repeatedly:
-//   1. grow the window from {0, 1, 2, 3}: each step adds the candidate -- a
-//      single qubit, or the qubits a gate at the front of some qubit's queue
-//      still misses -- that makes the most further gates runnable per added
-//      qubit, until the window holds TB qubits or nothing gains;
+//   1. grow the window from {0, 1, 2, 3}: each step adds a candidate -- the
+//      qubits a gate at the front of some qubit's queue still misses -- until
+//      the window holds TB qubits or nothing gains.  Greedy growth takes the
+//      candidate that makes the most further gates runnable per added qubit;
+//      the planner looks one decision further: for each of the kLookahead
+//      best candidates by that measure it completes the growth greedily and
+//      keeps the candidate whose completed window runs the most gates;
 //   2. take the runnable gates in dependency order (front-of-queue closure)
 //      and fuse them into blocks of <= max_k qubits with plan_greedy, so every
 //      block lies inside the window (plan_greedy ends by merging narrow blocks
 //      into commuting neighbours, absorb_blocks);
 //   3. emit the blocks as one window.
-// On 30-qubit random circuits of depth 200 (TB = 13) this applies ~31 gates
-// per sweep against ~21 for block-first grouping, with more (but cheaper per
-// gate) blocks per sweep.  The gates a window takes commute with every gate
+// On 30-qubit random circuits of depth 200 (TB = 13) this applies ~38 gates
+// per sweep (~31 with greedy growth) against ~21 for block-first grouping,
+// with more (but cheaper per gate) blocks per sweep.  The gates a window takes commute with every gate
 // left behind that precedes them in the list (they act on other qubits), so
 // the state is the circuit's own up to that reordering.
 #include "qsv_sched.h"
 
 #include <algorithm>
+#include <unordered_map>
 
 namespace qsv {
 
 namespace {
+
+// Growth candidates completed per step (1 = greedy growth).  On the 30-qubit
+// depth-200 benchmark circuit: 1 -> 288 windows, 3 -> 246, 5 -> 237, 8 -> 238.
+constexpr int kLookahead = 5;
 
 // Qubits a record acts on (the no-op CNOT(q, q) is kept on q so that it is
 // counted like every other record).
@@ -65,39 +73,165 @@ struct Segment {
     // (advanced past them), in a dependency order.  Runnable sets only grow
     // with w, so a candidate's gain is the closure of w | add continued from
     // the positions after w's own closure.
+    // A worklist of qubits whose queue front may have become runnable: a
+    // record blocked on another qubit's front is retried when that front
+    // advances.
     int64_t closure(uint64_t w, size_t *h, std::vector<int64_t> *out) const
     {
         int64_t cnt = 0;
-        bool changed = true;
-        while (changed) {
-            changed = false;
-            for (int q = 0; q < n; q++) {
-                if (!(w >> q & 1)) continue;
-                while (h[q] < queue[q].size()) {
-                    const int64_t g = queue[q][h[q]];
-                    const int *gq = &qs[3 * g];
-                    bool ok = true;
-                    for (int j = 0; j < nq[g] && ok; j++) {
-                        const int x = gq[j];
-                        ok = (w >> x & 1) && h[x] < queue[x].size() && queue[x][h[x]] == g;
-                    }
-                    if (!ok) break;
-                    for (int j = 0; j < nq[g]; j++) h[gq[j]]++;
-                    if (out) out->push_back(g);
-                    cnt++;
-                    changed = true;
+        int work[64], top = 0;
+        for (uint64_t m = w; m; m &= m - 1) work[top++] = __builtin_ctzll(m);
+        uint64_t queued = w;
+        while (top > 0) {
+            const int q = work[--top];
+            queued &= ~(uint64_t(1) << q);
+            while (h[q] < queue[q].size()) {
+                const int64_t g = queue[q][h[q]];
+                const int *gq = &qs[3 * g];
+                bool ok = true;
+                for (int j = 0; j < nq[g] && ok; j++) {
+                    const int x = gq[j];
+                    ok = (w >> x & 1) && h[x] < queue[x].size() && queue[x][h[x]] == g;
                 }
+                if (!ok) break;
+                for (int j = 0; j < nq[g]; j++) {
+                    const int x = gq[j];
+                    h[x]++;
+                    if (x != q && !(queued >> x & 1)) {
+                        queued |= uint64_t(1) << x;
+                        work[top++] = x;
+                    }
+                }
+                if (out) out->push_back(g);
+                cnt++;
             }
         }
         return cnt;
+    }
+
+    // Growth candidates of window w whose closure left the queue positions
+    // h: the qubits the front record of each queue outside w still misses,
+    // at most `room` of them; gain = further runnable records per qubit.
+    void candidates(uint64_t w, const size_t *h, int room, std::vector<std::pair<double, uint64_t>> &out) const
+    {
+        out.clear();
+        size_t hc[64];
+        for (int q = 0; q < n; q++) {
+            if ((w >> q & 1) || h[q] >= queue[q].size()) continue;
+            const int64_t g = queue[q][h[q]];
+            uint64_t add = 0;
+            for (int x = 0; x < nq[g]; x++) add |= uint64_t(1) << qs[3 * g + x];
+            add &= ~w;
+            const int k = __builtin_popcountll(add);
+            if (k == 0 || k > room) continue;
+            bool dup = false;
+            for (const auto &c : out) dup |= c.second == add;
+            if (dup) continue;
+            std::copy(h, h + n, hc);
+            out.push_back({(double)closure(w | add, hc, nullptr) / k, add});
+        }
+    }
+
+    // Greedy growth of w (h: positions after w's closure, advanced with it)
+    // to tb qubits or until no candidate gains.  The result depends on w
+    // alone (the positions follow from it), so memo maps every window passed
+    // on the way to the grown window, and a later growth that reaches one of
+    // them stops there.
+    uint64_t grow_greedy(uint64_t w, size_t *h, int tb, std::unordered_map<uint64_t, uint64_t> &memo) const
+    {
+        std::vector<std::pair<double, uint64_t>> cs;
+        std::vector<uint64_t> path;
+        uint64_t out = 0;
+        while (true) {
+            auto it = memo.find(w);
+            if (it != memo.end()) {
+                out = it->second;
+                break;
+            }
+            path.push_back(w);
+            if (__builtin_popcountll(w) >= tb) {
+                out = w;
+                break;
+            }
+            candidates(w, h, tb - __builtin_popcountll(w), cs);
+            double best = 0;
+            uint64_t add = 0;
+            for (const auto &c : cs)
+                if (c.first > best) {
+                    best = c.first;
+                    add = c.second;
+                }
+            if (add == 0) {
+                out = w;
+                break;
+            }
+            w |= add;
+            closure(w, h, nullptr);
+        }
+        for (uint64_t p : path) memo[p] = out;
+        return out;
+    }
+
+    // Growth with lookahead: at each step, complete the growth greedily from
+    // each of the `look` best candidates and take the one whose completed
+    // window runs the most records (ties: the better first step).
+    uint64_t grow(int tb, int look) const
+    {
+        uint64_t w = 0;
+        for (int q = 0; q < std::min(4, n); q++) w |= uint64_t(1) << q;
+        size_t h[64], h2[64];
+        std::copy(head.begin(), head.end(), h);
+        closure(w, h, nullptr);
+        std::vector<std::pair<double, uint64_t>> cs;
+        std::unordered_map<uint64_t, uint64_t> memo;     // greedy completions
+        std::unordered_map<uint64_t, int64_t> runs;      // records a window runs
+        while (__builtin_popcountll(w) < tb) {
+            candidates(w, h, tb - __builtin_popcountll(w), cs);
+            std::stable_sort(cs.begin(), cs.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+            if (cs.empty() || cs[0].first <= 0) break;
+            uint64_t add = cs[0].second;
+            if (look > 1) {
+                int64_t best = -1;
+                for (int c = 0; c < (int)cs.size() && c < look && cs[c].first > 0; c++) {
+                    const uint64_t wc = w | cs[c].second;
+                    uint64_t full;
+                    auto m = memo.find(wc);
+                    if (m != memo.end()) {
+                        full = m->second;
+                    } else {
+                        std::copy(h, h + n, h2);
+                        closure(wc, h2, nullptr);
+                        full = grow_greedy(wc, h2, tb, memo);
+                    }
+                    auto r = runs.find(full);
+                    int64_t got;
+                    if (r != runs.end()) {
+                        got = r->second;
+                    } else {
+                        std::copy(head.begin(), head.end(), h2);
+                        got = runs[full] = closure(full, h2, nullptr);
+                    }
+                    if (got > best) {
+                        best = got;
+                        add = cs[c].second;
+                    }
+                }
+            }
+            w |= add;
+            closure(w, h, nullptr);
+        }
+        return w;
     }
 };
 
 }  // namespace
 
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
-                std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only)
+                std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only,
+                const std::vector<uint64_t> *masks_in, std::vector<uint64_t> *masks_out)
 {
+    size_t mi = 0;
+    if (masks_out) masks_out->clear();
     if (max_k < 4 || max_k > 5) return fail(QSV_ERR_ARG, "window planning fuses blocks of 4 or 5 qubits");
     if (n < 4 || n > 62) return fail(QSV_ERR_ARG, "window planning needs 4..62 qubits");
     tb = std::min(tb, n);
@@ -139,39 +273,15 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
         std::vector<qsv_gate> sub;
         std::vector<PlanItem> blocks;
         while (left > 0) {
-            uint64_t w = 0;
-            for (int q = 0; q < std::min(4, n); q++) w |= uint64_t(1) << q;
-            size_t hw[64], hc[64];
-            std::copy(sg.head.begin(), sg.head.end(), hw);
-            sg.closure(w, hw, nullptr);
-            while (__builtin_popcountll(w) < tb) {
-                const int room = tb - __builtin_popcountll(w);
-                double best = 0;
-                uint64_t best_add = 0;
-                auto consider = [&](uint64_t add) {
-                    const int k = __builtin_popcountll(add);
-                    if (k == 0 || k > room) return;
-                    std::copy(hw, hw + n, hc);
-                    const double gain = (double)sg.closure(w | add, hc, nullptr) / k;
-                    if (gain > best) {
-                        best = gain;
-                        best_add = add;
-                    }
-                };
-                // a candidate gains only if the record at the front of one of
-                // its qubits needs nothing outside w | add
-                for (int q = 0; q < n; q++) {
-                    if ((w >> q & 1) || hw[q] >= sg.queue[q].size()) continue;
-                    const int64_t g = sg.queue[q][hw[q]];
-                    uint64_t add = 0;
-                    for (int x = 0; x < sg.nq[g]; x++) add |= uint64_t(1) << sg.qs[3 * g + x];
-                    add &= ~w;
-                    consider(add);
-                }
-                if (best_add == 0) break;
-                w |= best_add;
-                sg.closure(w, hw, nullptr);
+            size_t hc[64];
+            uint64_t w;
+            if (masks_in) {
+                if (mi >= masks_in->size()) return fail(QSV_ERR_ARG, "fewer window masks than windows");
+                w = (*masks_in)[mi++];
+            } else {
+                w = sg.grow(tb, kLookahead);
             }
+            if (masks_out) masks_out->push_back(w);
             taken.clear();
             std::copy(sg.head.begin(), sg.head.end(), hc);
             const int64_t got = sg.closure(w, hc, &taken);

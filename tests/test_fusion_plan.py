@@ -153,7 +153,8 @@ def test_window_plan_keeps_measurements_in_order():
 
 def test_window_plan_needs_fewer_sweeps_than_block_first_grouping():
     """30 qubits, depth 200 (the benchmark circuit): the window planner needs
-    at most 300 sweeps of 13 tile bits for its 9,000 gates; block-first
+    at most 250 sweeps of 13 tile bits for its 9,000 gates (288 with greedy
+    growth, no lookahead); block-first
     grouping of greedy k = 4 blocks (every run of consecutive blocks whose
     qubits above 3 number <= 9 shares a sweep) needs more than 400."""
     circ = C.random_layers(30, 200, 2025, haar_frac=0.1)
@@ -168,7 +169,7 @@ def test_window_plan_needs_fewer_sweeps_than_block_first_grouping():
         else:
             sweeps += 1
             cur = set(hi)
-    assert windows <= 300 < 400 < sweeps, (windows, sweeps)
+    assert windows <= 250 < 400 < sweeps, (windows, sweeps)
 
 
 @pytest.mark.parametrize("k", [2, 3, 4, 5])
