@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <thread>
 
@@ -28,6 +29,7 @@ struct qsv_costmodel {
     float *m = nullptr, *v = nullptr, *grad = nullptr;
     hipStream_t stream = nullptr;
     int64_t step = 0;
+    std::mutex mu;               // one call at a time (the scheduler scores candidates on threads)
 };
 
 namespace qsv {
@@ -427,6 +429,7 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
     struct Cand {
         int k = 0, window = 0, rc = QSV_OK;
         std::string err;                 // the planner's message (errors are per thread)
+        double total = 0;                // predicted device ms
         std::vector<PlanItem> plan;
         std::vector<int> win;
         std::vector<uint64_t> masks;     // window plans: each window's qubits
@@ -444,19 +447,7 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
             c.window = 1;
             cands.push_back(std::move(c));
         }
-    std::vector<std::thread> pool;
-    for (Cand &c : cands)
-        pool.emplace_back([&c, gates, ng, mats, &o, s]() {
-            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win, true,
-                                          nullptr, &c.masks)
-                            : plan_greedy(gates, ng, mats, c.k, o.exact, c.plan, true);
-            if (c.rc) c.err = qsv_last_error();
-        });
-    for (std::thread &t : pool) t.join();
-    double best_total = std::numeric_limits<double>::infinity();
-    Cand *best = nullptr;
-    for (Cand &c : cands) {
-        if (c.rc) return fail(c.rc, c.err);
+    auto score = [&](Cand &c) {
         // the DP sees the qubit lists as the run will widen them (the
         // matrices are widened later); it needs nothing else of a block
         std::vector<PlanItem> shape(c.plan.size());
@@ -473,11 +464,28 @@ int plan_learned(qsv_state *s, const qsv_gate *gates, int64_t ng, const double *
             }
         }
         std::vector<Launch> launches;
-        double total = 0;
-        int rc = group_launches(s, shape, o, launches, &total, c.window ? &c.win : nullptr);
-        if (rc) return rc;
-        if (total < best_total) {
-            best_total = total;
+        return group_launches(s, shape, o, launches, &c.total, c.window ? &c.win : nullptr);
+    };
+    // each thread plans and scores its candidate (feature building is host
+    // work; the model's inference calls take turns)
+    std::vector<std::thread> pool;
+    for (Cand &c : cands)
+        pool.emplace_back([&c, gates, ng, mats, &o, s, &score]() {
+            c.rc = c.window ? plan_window(gates, ng, mats, c.k, o.exact, kMaxPassTileBits, s->n, c.plan, c.win, true,
+                                          nullptr, &c.masks)
+                            : plan_greedy(gates, ng, mats, c.k, o.exact, c.plan, true);
+            // a new thread's current device is device 0: score on the state's
+            if (!c.rc && hipSetDevice(s->device) != hipSuccess) c.rc = fail(QSV_ERR_DEVICE, "cannot select the device");
+            if (!c.rc) c.rc = score(c);
+            if (c.rc) c.err = qsv_last_error();
+        });
+    for (std::thread &t : pool) t.join();
+    double best_total = std::numeric_limits<double>::infinity();
+    Cand *best = nullptr;
+    for (Cand &c : cands) {
+        if (c.rc) return fail(c.rc, c.err);
+        if (c.total < best_total) {
+            best_total = c.total;
             best = &c;
         }
     }
@@ -581,6 +589,7 @@ int qsv_cm_forward(qsv_costmodel *m, const float *x, int64_t batch, float *y)
 {
     if (!m || !x || !y || batch < 0) return fail(QSV_ERR_ARG, "bad argument");
     if (batch == 0) return QSV_OK;
+    std::lock_guard<std::mutex> lock(m->mu);
     Batch B;
     int rc = upload_batch(m, x, nullptr, batch, false, B);
     if (!rc) rc = forward(m, B);
@@ -593,6 +602,7 @@ int qsv_cm_forward(qsv_costmodel *m, const float *x, int64_t batch, float *y)
 int qsv_cm_gradients(qsv_costmodel *m, const float *x, const float *t, int64_t batch, float *grad, double *loss)
 {
     if (!m || !x || !t || !grad || batch < 1) return fail(QSV_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lock(m->mu);
     Batch B;
     int rc = upload_batch(m, x, t, batch, true, B);
     if (!rc) rc = forward(m, B);
@@ -608,6 +618,7 @@ int qsv_cm_train(qsv_costmodel *m, const float *x, const float *t, int64_t batch
                  double *loss)
 {
     if (!m || !x || !t || batch < 1 || epochs < 0) return fail(QSV_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lock(m->mu);
     Batch B;
     int rc = upload_batch(m, x, t, batch, true, B);
     if (rc) return rc;
