@@ -411,8 +411,12 @@ void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only)
     bool changed = true;
     while (changed) {
         changed = false;
-        for (size_t b = 0; b < plan.size(); b++) {
-            if (plan[b].barrier >= 0) continue;
+        size_t b = 0;
+        while (b < plan.size()) {
+            if (plan[b].barrier >= 0) {
+                b++;
+                continue;
+            }
             const uint64_t mb = qubit_mask(plan[b]);
             size_t best = b;
             int best_w = max_k + 1;
@@ -434,11 +438,13 @@ void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only)
                 }
                 if (mc & mb) break;
             }
-            if (best == b) continue;
+            if (best == b) {
+                b++;
+                continue;
+            }
             if (best > b) plan[best] = product_block(plan[b], plan[best], shape_only);
             else plan[best] = product_block(plan[best], plan[b], shape_only);
-            plan.erase(plan.begin() + (std::ptrdiff_t)b);
-            b--;                    // the next block moved into slot b
+            plan.erase(plan.begin() + (std::ptrdiff_t)b);   // the next block moves into slot b
             changed = true;
         }
     }
