@@ -106,6 +106,34 @@ def test_learned_schedule_does_not_change_results():
 
 
 @pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
+def test_learned_window_plans_across_measurements():
+    """The learned scheduler plans its candidates as shapes and rebuilds the
+    winner with matrices, replaying the window plan's qubit masks; with
+    MEASURE and RESET records the windows restart per segment.  Same random
+    stream as the plain FP64 path (one draw per record), same outcomes here,
+    and the state at the fused-path bar."""
+    n = 20
+    b = C._Builder(n)
+    for seed, rec in ((31, (C.MEASURE, 5, 0)), (32, (C.RESET, 11)), (33, None)):
+        for g in C.random_layers(n, 8, seed).gates:
+            b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+        if rec:
+            b.add(*rec)
+    circ = b.build()
+    out_f = np.zeros(8, np.int32)
+    out_p = np.zeros(8, np.int32)
+    with E.StateVector(n, E.C64) as f, E.StateVector(n, E.C128) as p:
+        f.attach_costmodel(E.CostModel.load())
+        st = f.run(circ, fuse_k=4, split=3, scheduler=1, rng_state=4242, outcomes=out_f)
+        p.run(circ, rng_state=4242, outcomes=out_p)
+        say("learned planner", st.planner, "k", st.fuse_k, "blocks", st.blocks, "outcomes", out_f[:2], out_p[:2])
+        assert st.planner == 1                   # window plans, rebuilt from replayed masks
+        assert f.last_rng == p.last_rng
+        assert np.array_equal(out_f[:2], out_p[:2])
+        assert f.fidelity(p) >= 1 - 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
 @pytest.mark.parametrize("split", [2, 3])
 def test_trained_model_gain_over_greedy(split):
     m = E.CostModel.load()
