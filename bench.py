@@ -94,8 +94,6 @@ def parse():
                    help="greedy: one block per sweep; passes: greedy multi-block sweeps; "
                         "learned: cost-model schedule (needs a trained model)")
     p.add_argument("--group", type=int, default=8, help="blocks per sweep for --scheduler passes")
-    p.add_argument("--pass-kernel", choices=["auto", "tile", "pipe"], default="auto",
-                   help="multi-block sweep kernel: one workgroup per tile or the pipelined one")
     p.add_argument("--pass-tb", type=int, default=0,
                    help="tile bits of multi-block sweeps (passes, window; 0 = minimal / 13)")
     p.add_argument("--costmodel", default=None, help="trained cost-model parameters (float32)")
@@ -110,7 +108,6 @@ def parse():
                    help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
     a = p.parse_args()
     a.split = {"fp32": 0, "split": 1, "bf16x6": 2, "fp16x3": 3}[a.product]
-    a.pk = {"auto": 0, "tile": 1, "pipe": 2}[a.pass_kernel]
     if a.n is None:
         a.n = {"large33": 33, "qrunes": 26, "stateio": 28}.get(a.workload, 30)
     if a.depth is None:
@@ -332,15 +329,14 @@ def main():
                                    "sweeps": int(greedy.launches), "fidelity_vs_fp64": sv_plain.fidelity(sv)}
         for _ in range(warmup):
             sv.reset()
-            sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group, pass_tb=a.pass_tb,
-                   pass_kernel=a.pk)
+            sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group, pass_tb=a.pass_tb)
         sv.sync()
         t0 = time.perf_counter()
         stats = []
         for _ in range(steps):
             sv.reset()
             stats.append(sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group,
-                                pass_tb=a.pass_tb, pass_kernel=a.pk, profile=True))
+                                pass_tb=a.pass_tb, profile=True))
         sv.sync()
         wall = time.perf_counter() - t0
         head_stats = stats

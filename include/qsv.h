@@ -291,11 +291,8 @@ typedef struct qsv_run_opts {
     int group;          /* scheduler 2: at most this many consecutive blocks    */
                         /* share one HBM sweep (0 or 1 = one block per sweep)   */
     int pass_tb;        /* scheduler 2: tile bits of such a pass (0 = minimal)  */
-    int pass_kernel;    /* multi-block sweeps: 0 = automatic (= 1), 1 = one     */
-                        /* workgroup per tile, 2 = pipelined (persistent, one   */
-                        /* workgroup per CU, HBM traffic of the neighbouring    */
-                        /* tiles overlapping the blocks; 64 KiB tiles of k = 4  */
-                        /* blocks with the BF16x6 / FP16x3 products)            */
+    int pass_kernel;    /* multi-block sweeps: 0 or 1 = one workgroup per tile  */
+                        /* (other values: QSV_ERR_UNSUPPORTED)                  */
 } qsv_run_opts;
 /* opts.scheduler: 0 = greedy blocks at fuse_k, one sweep each;
  *                 1 = learned (planner, block size, grouping into sweeps,

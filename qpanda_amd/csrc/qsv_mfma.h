@@ -35,9 +35,10 @@ typedef _Float16 f16x2v __attribute__((ext_vector_type(2)));
 // hf, element e <-> real column 2 (8 sb + e) + hf), so the B values of K-block
 // sb are this lane's FP32 B values bv[8 sb .. 8 sb + 7], already scaled.
 // `one` must be 1.0f but unknown to the compiler (a kernel argument): the
-// residual x * one - fp16(x) then compiles to one v_fma_mix_f32 that reads the
-// fp16 half in place, where x - fp16(x) with a literal costs a conversion
-// and a subtraction (16 VALU instructions more per 32-column chunk).
+// residual x * one - fp16(x), rounded to fp16, then compiles to one
+// v_fma_mixlo_f16 / v_fma_mixhi_f16 that reads the fp16 half in place and
+// writes the low piece's half in place, where x - fp16(x) with a literal
+// costs a conversion and a subtraction.
 template <int HALVES, int LOBLK>
 __device__ __forceinline__ void fp16x3_product(floatx16 (&acc)[HALVES], const f16x8 (&ap)[2][HALVES][LOBLK],
                                                const float *bv, float one)
@@ -49,12 +50,12 @@ __device__ __forceinline__ void fp16x3_product(floatx16 (&acc)[HALVES], const f1
         for (int e = 0; e < 8; e += 2) {
             const float x0 = bv[8 * sb + e], x1 = bv[8 * sb + e + 1];
             const f16x2v h = __builtin_convertvector((f32x2v){x0, x1}, f16x2v);
-            const f16x2v l = __builtin_convertvector(
-                (f32x2v){__builtin_fmaf(x0, one, -(float)h[0]), __builtin_fmaf(x1, one, -(float)h[1])}, f16x2v);
             b0[e] = h[0];
             b0[e + 1] = h[1];
-            b1[e] = l[0];
-            b1[e + 1] = l[1];
+            // the residual is exact in FP32; fma-mix rounds it to fp16 directly
+            // into either half of the register (v_fma_mixlo/hi_f16)
+            b1[e] = (_Float16)__builtin_fmaf(x0, one, -(float)h[0]);
+            b1[e + 1] = (_Float16)__builtin_fmaf(x1, one, -(float)h[1]);
         }
 #pragma unroll
         for (int h = 0; h < HALVES; h++) {

@@ -275,18 +275,25 @@ __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, con
     constexpr int PLANE = 1 << TB;                   // im plane offset (floats)
     const int hf = lane >> 5;
     // byte offsets: the swizzled index is linear, so an element's offset is
-    // the XOR of its row part, its column bits' parts and the plane bit
+    // the XOR of its row part, its column bits' parts and the plane bit.  The
+    // lane's masks (0 or -1 per lane bit) do not depend on the block, so the
+    // block loop keeps them; per block each column bit then costs one
+    // three-input bit operation.
     int elane = hf << (TB + 2);
 #pragma unroll
-    for (int i = 0; i < 5; i++)
-        if (lane >> i & 1) elane ^= 4 * pb.colp[i];
-    // rows this lane writes: 4 (j >> 1) + (j & 1) + 2 hf (arithmetic, not a
-    // select between array entries, which would index the array dynamically)
-    int wrow[8];
+    for (int i = 0; i < 5; i++) elane ^= (-((lane >> i) & 1)) & (4 * pb.colp[i]);
+    // rows this lane writes: 4 (j >> 1) + (j & 1) + 2 hf, i.e. the wave-uniform
+    // offset wa0[j] of row r0 = 4 (j >> 1) + (j & 1), XOR-ed with wdm[j] (the
+    // difference to row r0 + 2 in the lanes of the upper half); an indexed read
+    // of erow would index the array dynamically, and hf * (b - a) costs a
+    // quarter-rate multiply per row
+    const int hfmask = -hf;
+    int wa0[8], wdm[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const int r0 = 4 * (j >> 1) + (j & 1);
-        wrow[j] = 4 * (pb.erow[r0] + hf * (pb.erow[r0 + 2] - pb.erow[r0]));
+        wa0[j] = 4 * pb.erow[r0];
+        wdm[j] = (wa0[j] ^ (4 * pb.erow[r0 + 2])) & hfmask;
     }
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     char *const base = reinterpret_cast<char *>(pre);
@@ -332,7 +339,7 @@ __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, con
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int j = 2 * (v >> 2) + ((v & 3) >> 1);
-            *reinterpret_cast<float *>(base + (wrow[j] ^ wcol) + (v & 1) * 4 * PLANE) = acc[v];
+            *reinterpret_cast<float *>(base + (wa0[j] ^ wdm[j] ^ wcol) + (v & 1) * 4 * PLANE) = acc[v];
         }
     }
 }
@@ -409,137 +416,6 @@ __global__ __launch_bounds__(pass_threads(TB, ANY5)) void k_fused_pass(float2 *_
     fused_pass_body<TB, MODE, ANY5>(a, ntiles, pa);
 }
 
-// Pipelined sweep over 64 KiB tiles of k = 4 blocks.  One persistent
-// workgroup per CU owns two LDS tile buffers and two kinds of waves:
-//   compute waves (8): apply the pass's blocks to tile i in buffer i % 2,
-//                      a barrier after each block (as in k_fused_pass);
-//   memory waves  (8): meanwhile drain tile i - 1 from the other buffer to
-//                      HBM, stage tile i + 1 (loaded one iteration earlier)
-//                      into it and issue the loads of tile i + 2.
-// Every wave passes the same G barriers per tile (G = pa.nblocks, uniform),
-// the roles just run different code between them.  Splitting the roles keeps
-// the memory waves' outstanding loads out of the compute waves' vmcnt, so the
-// operand loads of a block never wait for the next tile's HBM loads.  A tile
-// would take max(G blocks of compute, one HBM read + write) instead of their
-// sum, but the blocks are bound by VALU and LDS issue, and with 2 compute
-// waves per SIMD (against 4 in k_fused_pass with 512 threads) they run ~2x
-// slower: at 30 qubits, windows of ~5.8 k = 4 blocks, 6.1 ms per sweep
-// against 4.9 ms for k_fused_pass, so it is not the automatic choice.
-constexpr int kPipeCompute = 8, kPipeMem = 8;     // waves per role
-constexpr int kPipeThreads = 64 * (kPipeCompute + kPipeMem);
-constexpr int kPipeMemThreads = 64 * kPipeMem;
-
-template <int MODE>
-__global__ __launch_bounds__(kPipeThreads) void k_fused_pass_pipe(float2 *__restrict__ a, uint64_t ntiles,
-                                                                  PassArgs pa)
-{
-    constexpr int TB = 13, TSIZE = 1 << TB;
-    constexpr int NLD = TSIZE / (2 * kPipeMemThreads);  // 16-byte loads per memory thread
-    __shared__ float planes[2][2][TSIZE];             // [buffer][re, im][element]
-    // the role branch must be wave-uniform (a scalar branch): a wave must never
-    // step through the other role's barriers with its lanes masked off
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint64_t first = blockIdx.x;
-    if (first >= ntiles) return;                      // whole workgroup
-    const uint64_t T = (ntiles - 1 - first) / gridDim.x + 1;   // tiles of this workgroup
-    const int G = pa.nblocks;
-    if (wave >= kPipeCompute) {
-        const int mt = tid - 64 * kPipeCompute;       // 0 .. 511
-        // element 2 mt + 1024 j of a tile sits at base + pdep(2 mt, L) + pdep(1024 j, L)
-        static_assert(kPipeMemThreads == 512, "memory-thread index bits");
-        uint64_t toff = 0;
-#pragma unroll
-        for (int j = 1; j <= 9; j++)
-            if ((mt >> (j - 1)) & 1) toff |= uint64_t(1) << pa.lpos[j];
-        const int sw = lds_swizzle(2 * mt) ^ (2 * mt);
-        const float ls = MODE == 3 ? pa.bscale : 1.0f, li = MODE == 3 ? pa.binv : 1.0f;   // as k_fused_pass
-        // two register sets: the loads of tile i + 2 are issued while tile i
-        // is computed and staged one iteration later, so they have a whole
-        // tile period to arrive
-        float4 bufA[NLD], bufB[NLD];
-        auto base_of = [&](uint64_t i) { return expand_index(first + i * gridDim.x, pa.tile_ins) + toff; };
-        auto load = [&](float4 (&buf)[NLD], uint64_t i) {
-            const uint64_t b = base_of(i);
-#pragma unroll
-            for (int j = 0; j < NLD; j++) buf[j] = *reinterpret_cast<const float4 *>(a + b + pa.hioff[j]);
-        };
-        auto stage = [&](const float4 (&buf)[NLD], int bi) {
-            float *pre = planes[bi][0], *pim = planes[bi][1];
-#pragma unroll
-            for (int j = 0; j < NLD; j++) {
-                const int e = (2 * (mt + kPipeMemThreads * j)) ^ sw;
-                *reinterpret_cast<float2 *>(pre + e) = make_float2(buf[j].x * ls, buf[j].z * ls);
-                *reinterpret_cast<float2 *>(pim + e) = make_float2(buf[j].y * ls, buf[j].w * ls);
-            }
-        };
-        auto drain = [&](int bi, uint64_t i) {
-            const float *pre = planes[bi][0], *pim = planes[bi][1];
-            const uint64_t b = base_of(i);
-#pragma unroll
-            for (int j = 0; j < NLD; j++) {
-                const int e = (2 * (mt + kPipeMemThreads * j)) ^ sw;
-                const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
-                float4 v;
-                v.x = r.x * li; v.y = m.x * li;
-                v.z = r.y * li; v.w = m.y * li;
-                *reinterpret_cast<float4 *>(a + b + pa.hioff[j]) = v;
-            }
-        };
-        load(bufA, 0);
-        if (T > 1) load(bufB, 1);
-        stage(bufA, 0);
-        __syncthreads();
-        // iterations in pairs so that each register set has a fixed role:
-        // even i computes in buffer 0, drains tile i - 1 from buffer 1, stages
-        // tile i + 1 (bufB) into it and loads tile i + 2 into bufA; odd i the
-        // mirror image.  A thread drains and restages only its own elements.
-        for (uint64_t i = 0; i < T; i += 2) {
-            for (int g = 0; g < G; g++) {
-                if (g == 0) {
-                    if (i > 0) drain(1, i - 1);
-                    if (i + 1 < T) stage(bufB, 1);
-                    if (i + 2 < T) load(bufA, i + 2);
-                }
-                __syncthreads();
-            }
-            if (i + 1 >= T) break;
-            for (int g = 0; g < G; g++) {
-                if (g == 0) {
-                    drain(0, i);
-                    if (i + 2 < T) stage(bufA, 0);
-                    if (i + 3 < T) load(bufB, i + 3);
-                }
-                __syncthreads();
-            }
-        }
-        drain((int)((T - 1) & 1), T - 1);
-    } else {
-        __syncthreads();
-        for (uint64_t i = 0; i < T; i++) {
-            float *pre = planes[i & 1][0];
-            for (int g = 0; g < G; g++) {
-                Ops4<MODE> ops;
-                load_ops4<MODE>(ops, pa.b[g].mops, lane);
-                pass_apply4<TB, MODE, kPipeCompute>(pre, pa.b[g], ops, lane, wave, pa.one, 1.0f);
-                __syncthreads();
-            }
-        }
-    }
-}
-
-// Workgroups of the pipelined sweep: one per CU of the state's device.
-static unsigned pipe_grid(qsv_state *s)
-{
-    static int cus[64] = {0};
-    const int d = s->device >= 0 && s->device < 64 ? s->device : 0;
-    if (!cus[d]) {
-        int c = 0;
-        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || c <= 0) c = 256;
-        cus[d] = c;
-    }
-    return (unsigned)cus[d];
-}
-
 // Tile bits a pass over these (widened) blocks needs: qubits 0..3 (the
 // contiguous run) plus every block qubit above them, at least 11 and at most
 // 13 (64 KiB of LDS).  -1 when they do not fit.
@@ -573,9 +449,7 @@ static void launch_tb(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int spl
 }
 
 // blocks[g]: widened qubit list (4 or 5 qubits) and device operand pointer.
-// kernel: 0 automatic (one workgroup per tile), 1 one workgroup per tile,
-// 2 the pipelined sweep where it applies (64 KiB tiles of k = 4 blocks with
-// the BF16x6 or FP16x3 product).
+// kernel: 0 or 1 (one workgroup per tile, the only sweep kernel).
 int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
                     const std::vector<const float *> &mops, int split, int tb, int kernel)
 {
@@ -584,6 +458,7 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     if (G < 1 || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..16 blocks per pass");
     if (tb < 11 || tb > kMaxPassTileBits || tb > s->n) return fail(QSV_ERR_ARG, "pass tile bits must be 11..13");
     if (split < 0 || split > 3) return fail(QSV_ERR_ARG, "split must be 0, 1, 2 or 3");
+    if (kernel < 0 || kernel > 1) return fail(QSV_ERR_UNSUPPORTED, "pass kernel must be 0 or 1");
     uint64_t u = 0;
     for (auto *q : qsets) {
         if (q->size() != 4 && q->size() != 5) return fail(QSV_ERR_ARG, "pass blocks must have 4 or 5 qubits");
@@ -611,8 +486,7 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     };
     bool any5 = false;
     for (auto *q : qsets) any5 |= q->size() == 5;
-    const bool pipe = kernel == 2 && tb == 13 && !any5 && (split == 2 || split == 3);
-    const int nt = pipe ? kPipeMemThreads : pass_threads(tb, any5);   // threads that move tiles
+    const int nt = pass_threads(tb, any5);
     for (int i = 0; i < (1 << tb) / (2 * nt); i++) pa.hioff[i] = pdep((uint64_t)(2 * nt) * i);
     auto rank = [&](int bit) {
         for (int i = 0; i < tb; i++)
@@ -640,15 +514,6 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
         }
     }
     const uint64_t ntiles = s->dim >> tb;
-    if (pipe) {
-        const dim3 g((unsigned)std::min<uint64_t>(ntiles, pipe_grid(s))), t(kPipeThreads);
-        float2 *a = (float2 *)s->amps;
-        if (split == 3) hipLaunchKernelGGL(k_fused_pass_pipe<3>, g, t, 0, s->stream, a, ntiles, pa);
-        else hipLaunchKernelGGL(k_fused_pass_pipe<2>, g, t, 0, s->stream, a, ntiles, pa);
-        s->launches++;
-        QSV_HIP(hipGetLastError());
-        return QSV_OK;
-    }
     switch (tb) {
     case 11: launch_tb<11>(s, ntiles, pa, split); break;
     case 12: launch_tb<12>(s, ntiles, pa, split); break;

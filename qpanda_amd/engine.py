@@ -347,7 +347,7 @@ class StateVector:
         """scheduler: 0 greedy blocks (one sweep each), 1 learned (needs an attached
         cost model), 2 greedy blocks grouped into multi-block sweeps of <= group,
         3 window planner (one sweep per window of pass_tb tile bits, 0 = 13).
-        pass_kernel: 0 automatic, 1 one workgroup per tile, 2 pipelined sweeps."""
+        pass_kernel: 0 or 1 (one workgroup per tile)."""
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
         opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile),

@@ -211,19 +211,15 @@ def test_passes_with_measurements_keep_rng_accounting():
 @pytest.mark.parametrize("pass_tb", [0, 12])
 @pytest.mark.parametrize("k", [4, 5])
 @pytest.mark.parametrize("split", [2, 3])
-@pytest.mark.parametrize("pass_kernel", [1, 2])
-def test_window_scheduler_vs_oracle(pass_tb, k, split, pass_kernel):
+def test_window_scheduler_vs_oracle(pass_tb, k, split):
     """Window planner (scheduler 3): one sweep per window, every block inside
-    it; the state matches the FP64 oracle at the fused-path bar, with the
-    one-workgroup-per-tile sweep (1) and the pipelined one (2, used for
-    13-bit windows of k = 4 blocks)."""
+    it; the state matches the FP64 oracle at the fused-path bar."""
     n = 18
     circ = C.random_layers(n, 16, 700 + k + pass_tb, haar_frac=0.1)
     ref, _ = O.run_circuit(circ)
-    say("window scheduler tb", pass_tb, "k", k, "split", split, "kernel", pass_kernel)
+    say("window scheduler tb", pass_tb, "k", k, "split", split)
     with E.StateVector(n, E.C64) as sv:
-        st = sv.run(circ, fuse_k=k, split=split, scheduler=3, pass_tb=pass_tb, pass_kernel=pass_kernel,
-                    profile=True)
+        st = sv.run(circ, fuse_k=k, split=split, scheduler=3, pass_tb=pass_tb, profile=True)
         got = sv.get()
     assert st.planner == 1 and st.fuse_k == k
     windows = len({w for *_, w in E.window_plan(circ, k, pass_tb or 13)})
@@ -255,23 +251,6 @@ def test_window_scheduler_with_measurements_keeps_rng_accounting():
         assert f.last_rng == p.last_rng          # one draw per record, as the reference
         if out_f[0] == out_p[0]:
             assert f.fidelity(p) >= 1 - 1e-6
-
-
-@pytest.mark.parametrize("n", [13, 14, 22])
-@pytest.mark.parametrize("split", [2, 3])
-def test_pipelined_sweep_matches_tile_sweep(n, split):
-    """The pipelined sweep (persistent workgroups, one to every CU, tiles in
-    turn) and the one-workgroup-per-tile sweep give the same state: one tile
-    (n = 13), fewer tiles than CUs (14) and several tiles per workgroup (22)."""
-    circ = C.random_layers(n, 8, 4400 + n, haar_frac=0.1)
-    outs = []
-    for kern in (1, 2):
-        with E.StateVector(n, E.C64) as sv:
-            st = sv.run(circ, fuse_k=4, split=split, scheduler=3, pass_kernel=kern, profile=True)
-            outs.append(sv.get())
-        assert st.classes().get("pass", {}).get("launches", 0) > 0
-    say("pipelined vs tile sweep n", n, "max diff", np.abs(outs[0] - outs[1]).max())
-    assert np.abs(outs[0] - outs[1]).max() < 1e-6
 
 
 @pytest.mark.parametrize("n", [8, 10, 11, 12, 13, 14])
