@@ -259,8 +259,12 @@ def random_gaussian_state(n: int, seed: int) -> np.ndarray:
     return psi
 
 
-def all_gate_kinds(n: int, seed: int, reps: int = 3) -> Circuit:
-    """Every uncontrolled gate kind, both dagger settings, random qubits/angles."""
+def all_gate_kinds(n: int, seed: int, reps: int = 3, u1: bool = False) -> Circuit:
+    """Every uncontrolled gate kind, both dagger settings, random qubits/angles.
+    u1: also general 2x2 records (U1, a Haar-random unitary each); the
+    reference cannot run them (its unitarySingleQubitGate addresses qubit
+    qn - 1, CPUQuantumGates.cpp:1188), so the fixtures made by the reference
+    leave them out."""
     rng = Xoshiro256(seed)
     b = _Builder(n)
 
@@ -292,4 +296,7 @@ def all_gate_kinds(n: int, seed: int, reps: int = 3) -> Circuit:
         b.add(TOFFOLI, a, c, t)
         a, c, t = pick(3)
         b.add(CSWAP, a, c, t)
-    return b.build(f"allkinds_n{n}_s{seed}")
+        if u1:
+            for _ in range(2):
+                b.add(U1, pick(1)[0], mat=haar_unitary(rng, 2))
+    return b.build(f"allkinds_n{n}_s{seed}" + ("_u1" if u1 else ""))

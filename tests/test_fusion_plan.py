@@ -36,7 +36,7 @@ def test_plan_reproduces_random_circuit(k):
 @pytest.mark.parametrize("k", [3, 4, 5])
 @pytest.mark.parametrize("exact", [False, True])
 def test_plan_every_gate_kind(k, exact):
-    circ = C.all_gate_kinds(8, 5 + k, reps=4)
+    circ = C.all_gate_kinds(8, 5 + k, reps=4, u1=True)
     psi = C.random_gaussian_state(8, 50 + k)
     ref = O.State(8, exact=exact)
     ref.set(psi)
@@ -112,7 +112,7 @@ def test_window_plan_reproduces_random_circuit(k, tb):
 @pytest.mark.parametrize("exact", [False, True])
 def test_window_plan_every_gate_kind(exact):
     n = 9
-    circ = C.all_gate_kinds(n, 21, reps=4)
+    circ = C.all_gate_kinds(n, 21, reps=4, u1=True)
     psi = C.random_gaussian_state(n, 77)
     ref = O.State(n, exact=exact)
     ref.set(psi)

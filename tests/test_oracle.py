@@ -360,3 +360,103 @@ def test_measure_collapse_renormalises():
     assert out == int(np.float32(0.999) > p0)
     assert abs(st.norm2() - 1) < 1e-12
     assert np.all(st.psi[(np.arange(1 << n) >> 3 & 1) != out] == 0)
+
+
+# ------------------------------------------- general 2x2 and exact modes ----
+def _random_2x2(g, unitary):
+    if unitary:
+        return C.haar_unitary(C.Xoshiro256(int(g.integers(1 << 30))), 2)
+    return g.standard_normal((2, 2)) + 1j * g.standard_normal((2, 2))
+
+
+@pytest.mark.parametrize("unitary", [True, False])
+def test_u1_matches_kronecker_and_k1_block(unitary):
+    """orc_u1 (the general 2x2, a correct unitarySingleQubitGate; the
+    reference's, CPUQuantumGates.cpp:1170-1212, addresses qubit qn - 1) against
+    the dense Kronecker operator and the independent k = 1 block loop, on
+    unitary and on general complex matrices (a transposed or conjugated
+    entry fails both)."""
+    g = np.random.default_rng(31 + unitary)
+    n = 6
+    for q in range(n):
+        u = _random_2x2(g, unitary)
+        psi = C.random_gaussian_state(n, 40 + q)
+        a = O.State(n); a.set(psi); a.u1(q, u)
+        assert np.abs(a.psi - kron_op(n, [q], u) @ psi).max() < 1e-12
+        b = O.State(n); b.set(psi); b.block([q], u)
+        assert np.abs(a.psi - b.psi).max() < 1e-12
+
+
+def test_controlled_u1_matches_masked_matrix():
+    """Controlled U1 (orc_ctrl_1q op 5, controls then target as in
+    Instruction.cpp:62) against the brute-force controlled matrix."""
+    g = np.random.default_rng(77)
+    n = 6
+    for trial in range(12):
+        m = int(g.integers(1, 4))
+        qs = g.permutation(n)[:m + 1].tolist()
+        u = _random_2x2(g, trial % 2 == 0)
+        psi = C.random_gaussian_state(n, 500 + trial)
+        st = O.State(n); st.set(psi)
+        st.controlled("U1", qs, u=u)
+        exp = apply_controlled_dense(psi, n, qs[:-1], qs[-1], u)
+        assert np.abs(st.psi - exp).max() < 1e-12
+
+
+def test_exact_mode_cz_sqiswap_h_closed_forms():
+    """The ground-truth mode's exact gates against Kronecker operators of
+    their closed-form matrices: CZ = diag(1, 1, 1, -1); sqiSWAP with 1/sqrt(2)
+    and +i (dagger -i) on {|01>, |10>} (CPUQuantumGates.cpp:978-1039 with the
+    exact constant); H with 1/sqrt(2).  The same records run through the
+    oracle's circuit runner in exact mode."""
+    n = 5
+    r = 1 / math.sqrt(2)
+    cz = np.diag([1, 1, 1, -1]).astype(complex)
+    for a in range(n):
+        for b in range(n):
+            if a == b:
+                continue
+            psi = C.random_gaussian_state(n, 10 * a + b)
+            st = O.State(n, exact=True); st.set(psi); st.cz_exact(a, b)
+            assert np.abs(st.psi - kron_op(n, [a, b], cz) @ psi).max() < 1e-13
+            for dag in (0, 1):
+                k = -1j if dag else 1j
+                sq = np.array([[1, 0, 0, 0], [0, r, k * r, 0], [0, k * r, r, 0], [0, 0, 0, 1]], complex)
+                st = O.State(n, exact=True); st.set(psi); st.sqiswap(a, b, dag)
+                assert np.abs(st.psi - kron_op(n, [a, b], sq) @ psi).max() < 1e-13
+    h = np.array([[r, r], [r, -r]], complex)
+    psi = C.random_gaussian_state(n, 99)
+    st = O.State(n, exact=True); st.set(psi); st.h(2)
+    assert np.abs(st.psi - kron_op(n, [2], h) @ psi).max() < 1e-13
+    # circuit records in exact mode: CZ(1,3), SQISWAP(0,4), SQISWAP+(2,1), H(3)
+    b = C._Builder(n)
+    b.add(C.CZ, 1, 3)
+    b.add(C.SQISWAP, 0, 4)
+    b.add(C.SQISWAP, 2, 1, dagger=1)
+    b.add(C.H, 3)
+    st = O.State(n, exact=True); st.set(psi); st.run(b.build())
+    sq = lambda k: np.array([[1, 0, 0, 0], [0, r, k * r, 0], [0, k * r, r, 0], [0, 0, 0, 1]], complex)
+    exp = kron_op(n, [3], h) @ (kron_op(n, [2, 1], sq(-1j)) @ (kron_op(n, [0, 4], sq(1j)) @ (kron_op(n, [1, 3], cz) @ psi)))
+    assert np.abs(st.psi - exp).max() < 1e-13
+    # plain mode: CZ runs as CR(pi), which leaves the sin(pi) residue the
+    # exact mode removes
+    st2 = O.State(n); st2.set(psi); st2.run(b.build())
+    assert 1e-17 < np.abs(st2.psi - st.psi).max() < 1e-12
+
+
+def test_u1_records_through_the_circuit_runner():
+    """U1 records (matrix pool entries) in the oracle's circuit runner equal
+    the Kronecker chain of their matrices."""
+    n = 5
+    circ = C.all_gate_kinds(n, 8, reps=1, u1=True)
+    sel = circ.gates["kind"] == C.U1
+    assert sel.sum() == 2
+    b = C._Builder(n)
+    for g in circ.gates[sel]:
+        b.add(C.U1, int(g["q0"]), mat=circ.mats[g["mat"]:g["mat"] + 4].reshape(2, 2))
+    psi = C.random_gaussian_state(n, 3)
+    st = O.State(n); st.set(psi); st.run(b.build())
+    exp = psi
+    for g in circ.gates[sel]:
+        exp = kron_op(n, [int(g["q0"])], circ.mats[g["mat"]:g["mat"] + 4].reshape(2, 2)) @ exp
+    assert np.abs(st.psi - exp).max() < 1e-12
