@@ -247,6 +247,26 @@ int qsv_pmeasure(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *
 /* sum_{i=0}^{n-2} <Z_i Z_{i+1}> in FP64. */
 int qsv_expect_zz_chain(qsv_state *s, double *value);
 
+/* Allocation-free form of qsv_norm2 / qsv_prob0 / qsv_expect_zz_chain.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  amps: device pointer to 2^n interleaved complex amplitudes of
+ * `precision`; out: device pointer to one double that receives the FP64 result
+ * (the same value the handle-based call returns: same kernels, same
+ * summation order).  Both must be device memory of the current device.  The
+ * kernels are queued on `stream` (a hipStream_t; NULL = the null stream) and
+ * the call returns without waiting.  It calls no hipMalloc / hipFree: the
+ * partial sums (at most 4096 doubles, 32 KiB) live in a library-owned device
+ * array, one per GPU, located with hipGetSymbolAddress on the first call on
+ * that device.  Calls that overlap on one GPU (other streams, other host
+ * threads) race on that array; the array keeps the last call's partial sums.
+ * what: QSV_READ_NORM2, QSV_READ_PROB0 (q = the qubit) or QSV_READ_ZZ_CHAIN.
+ * n = 1..34. */
+#define QSV_READ_NORM2    0
+#define QSV_READ_PROB0    1
+#define QSV_READ_ZZ_CHAIN 2
+int qsv_readout_noalloc(const void *amps, int n, int precision, int what, int q, double *out, void *stream);
+
 /* Shot loop of QPandaSDK::run (QPandaAPI.cpp:131-211, QuantumInstructionHandleAPI.cpp:96-114)
  * without re-simulating the program per shot.  The reference restarts every
  * shot from |0...0>, runs the program (one draw per gate call) and then measures;

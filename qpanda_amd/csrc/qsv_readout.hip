@@ -12,6 +12,7 @@
 #include "qsv_internal.h"
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 
 namespace qsv {
@@ -238,35 +239,56 @@ __global__ __launch_bounds__(kBlock) void k_marg_final(const double *__restrict_
     if (threadIdx.x == 0) out[o] = s;
 }
 
+// Partition of a reduction: nblk <= kRedParts workgroups of `chunk`
+// consecutive (compact) indices each, a fixed function of the size.
+constexpr uint64_t kRedParts = 4096;
+
+struct RedPlan {
+    InsertSet ins;
+    uint64_t count, chunk, nblk;
+};
+
+RedPlan reduce_plan(int n, int mode, int q)
+{
+    RedPlan r{};
+    r.count = uint64_t(1) << n;
+    if (mode == RED_PROB0) {
+        r.ins.n = 1;
+        r.ins.pos[0] = q;
+        r.count >>= 1;
+    }
+    uint64_t nblk = (r.count + (uint64_t)kBlock * 16 - 1) / ((uint64_t)kBlock * 16);
+    if (nblk > kRedParts) nblk = kRedParts;
+    if (nblk == 0) nblk = 1;
+    uint64_t chunk = (r.count + nblk - 1) / nblk;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    r.chunk = chunk;
+    r.nblk = (r.count + chunk - 1) / chunk;
+    return r;
+}
+
+// Partial sums into partial[0 .. nblk), their sum into *out (device), on st.
+template <typename T>
+void reduce_launch_t(const T *a, int n, int mode, const RedPlan &r, double *partial, double *out, hipStream_t st)
+{
+    const dim3 g((unsigned)r.nblk), t(kBlock);
+    if (mode == RED_PROB0) hipLaunchKernelGGL((k_reduce<T, RED_PROB0>), g, t, 0, st, a, r.count, r.chunk, r.ins, n, partial);
+    else if (mode == RED_ZZ) hipLaunchKernelGGL((k_reduce<T, RED_ZZ>), g, t, 0, st, a, r.count, r.chunk, r.ins, n, partial);
+    else hipLaunchKernelGGL((k_reduce<T, RED_NORM>), g, t, 0, st, a, r.count, r.chunk, r.ins, n, partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, st, partial, (int)r.nblk, out);
+}
+
+void reduce_launch(const void *amps, int prec, int n, int mode, int q, double *partial, double *out, hipStream_t st)
+{
+    const RedPlan r = reduce_plan(n, mode, q);
+    if (prec == QSV_C128) reduce_launch_t((const c128 *)amps, n, mode, r, partial, out, st);
+    else reduce_launch_t((const c64 *)amps, n, mode, r, partial, out, st);
+}
+
 int reduce_common(qsv_state *s, int mode, int q, double *result)
 {
-    InsertSet ins{};
-    uint64_t count = s->dim;
-    if (mode == RED_PROB0) {
-        ins.n = 1;
-        ins.pos[0] = q;
-        count = s->dim >> 1;
-    }
-    uint64_t nblk = (count + (uint64_t)kBlock * 16 - 1) / ((uint64_t)kBlock * 16);
-    if (nblk > 4096) nblk = 4096;
-    if (nblk == 0) nblk = 1;
-    uint64_t chunk = (count + nblk - 1) / nblk;
-    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
-    nblk = (count + chunk - 1) / chunk;
-    double *partial = s->red;
     double *out = s->red + (kRedSlots - 1);
-    if (s->prec == QSV_C128) {
-        const c128 *a = (const c128 *)s->amps;
-        if (mode == RED_PROB0) hipLaunchKernelGGL((k_reduce<c128, RED_PROB0>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
-        else if (mode == RED_ZZ) hipLaunchKernelGGL((k_reduce<c128, RED_ZZ>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
-        else hipLaunchKernelGGL((k_reduce<c128, RED_NORM>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
-    } else {
-        const c64 *a = (const c64 *)s->amps;
-        if (mode == RED_PROB0) hipLaunchKernelGGL((k_reduce<c64, RED_PROB0>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
-        else if (mode == RED_ZZ) hipLaunchKernelGGL((k_reduce<c64, RED_ZZ>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
-        else hipLaunchKernelGGL((k_reduce<c64, RED_NORM>), dim3(nblk), dim3(kBlock), 0, s->stream, a, count, chunk, ins, s->n, partial);
-    }
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s->stream, partial, (int)nblk, out);
+    reduce_launch(s->amps, s->prec, s->n, mode, q, s->red, out, s->stream);
     s->launches += 2;
     QSV_HIP(hipGetLastError());
     QSV_HIP(hipMemcpyAsync(s->host_red, out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -275,11 +297,78 @@ int reduce_common(qsv_state *s, int mode, int q, double *result)
     return QSV_OK;
 }
 
+static_assert(kRedParts < (uint64_t)kRedSlots, "state scratch holds the partials and the result");
+
+// ---- allocation-free readout -------------------------------------------------
+// Partial sums of qsv_readout_noalloc: one library-owned array per GPU (each
+// device holds its own copy until the process ends).  Host code never uses the
+// array's name as an address: its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) double scratch[kRedParts];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64] = {};
+
+int scratch_on_current_device(int *dev_out, double **ptr)
+{
+    int dev = 0;
+    QSV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(QSV_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *dev_out = dev;
+    *ptr = g_scratch_ptr[dev];
+    return QSV_OK;
+}
+
 }  // namespace
 
 int reduce_prob0(qsv_state *s, int q, double *p0) { return reduce_common(s, RED_PROB0, q, p0); }
 int reduce_norm2(qsv_state *s, double *v) { return reduce_common(s, RED_NORM, 0, v); }
 int reduce_zz_chain(qsv_state *s, double *v) { return reduce_common(s, RED_ZZ, 0, v); }
+
+}  // namespace qsv
+
+extern "C" int qsv_readout_noalloc(const void *amps, int n, int precision, int what, int q, double *out,
+                                   void *stream)
+{
+    using namespace qsv;
+    if (!amps || !out) return fail(QSV_ERR_ARG, "null amplitude or output pointer");
+    if (n < 1 || n > QSV_MAX_QUBITS) return fail(QSV_ERR_ARG, "qubit count must be 1..34");
+    if (precision != QSV_C128 && precision != QSV_C64) return fail(QSV_ERR_ARG, "precision must be QSV_C128 or QSV_C64");
+    int mode;
+    switch (what) {
+    case QSV_READ_NORM2: mode = RED_NORM; break;
+    case QSV_READ_PROB0: mode = RED_PROB0; break;
+    case QSV_READ_ZZ_CHAIN: mode = RED_ZZ; break;
+    default: return fail(QSV_ERR_ARG, "what must be QSV_READ_NORM2, QSV_READ_PROB0 or QSV_READ_ZZ_CHAIN");
+    }
+    if (mode == RED_PROB0 && (q < 0 || q >= n)) return fail(QSV_ERR_QUBIT, "qubit out of range");
+    const RedPlan r = reduce_plan(n, mode, q);
+    if (r.nblk > kRedParts) return fail(QSV_ERR_ARG, "reduction needs more partial sums than the scratch holds");
+    int dev = 0;
+    double *partial = nullptr;
+    int rc = scratch_on_current_device(&dev, &partial);
+    if (rc) return rc;
+    // the amplitudes and the result must live on the current device
+    hipPointerAttribute_t pa{}, po{};
+    if (hipPointerGetAttributes(&pa, amps) != hipSuccess || hipPointerGetAttributes(&po, out) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(QSV_ERR_ARG, "amplitudes and result must be device memory");
+    }
+    if (pa.type != hipMemoryTypeDevice || po.type != hipMemoryTypeDevice || pa.device != dev || po.device != dev)
+        return fail(QSV_ERR_ARG, "amplitudes and result must be device memory of the current device");
+    reduce_launch(amps, precision, n, mode, q, partial, out, (hipStream_t)stream);
+    QSV_HIP(hipGetLastError());
+    return QSV_OK;
+}
+
+namespace qsv {
 
 int inner(qsv_state *a, qsv_state *b, double *re, double *im)
 {

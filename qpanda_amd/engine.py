@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqsv.so")
 
 C128, C64 = 0, 1
+READ_NORM2, READ_PROB0, READ_ZZ_CHAIN = 0, 1, 2   # qsv_readout_noalloc
 FILE_BINARY, FILE_TEXT = 0, 1
 
 EXPORTED = [
@@ -30,7 +31,7 @@ EXPORTED = [
     "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
     "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
     "qsv_timing_log", "qsv_batch_create", "qsv_batch_destroy", "qsv_batch_autotune", "qsv_batch_run",
-    "qsv_batch_graph_run",
+    "qsv_batch_graph_run", "qsv_readout_noalloc",
 ]
 
 CM_FEATURES = 12
@@ -115,6 +116,7 @@ def lib():
             "qsv_pmeasure": ([vp, C.POINTER(I), I, np.ctypeslib.ndpointer(dtype=np.uint64), P,
                               C.POINTER(I)], I),
             "qsv_expect_zz_chain": ([vp, C.POINTER(D)], I),
+            "qsv_readout_noalloc": ([vp, I, I, I, I, vp, vp], I),
             "qsv_run_circuit": ([vp, vp, C.c_int64, vp, C.POINTER(RunOpts), C.POINTER(C.c_int32),
                                  vp, C.POINTER(RunStats)], I),
             "qsv_sample_shots": ([vp, C.POINTER(I), I, C.c_int64, C.POINTER(C.c_int32), C.c_int64,
@@ -300,6 +302,12 @@ class StateVector:
         v = C.c_double()
         _check(lib().qsv_expect_zz_chain(self.h, C.byref(v)))
         return v.value
+
+    def readout_noalloc(self, what: int, out_ptr: int, q: int = 0) -> None:
+        """Queue qsv_readout_noalloc on this state's stream; the FP64 result
+        lands in the device double at out_ptr (no allocation in the call)."""
+        _check(lib().qsv_readout_noalloc(self.device_ptr(), self.n, self.precision, int(what), int(q),
+                                         C.c_void_p(out_ptr), C.c_void_p(self.stream())))
 
     def collapse(self, q, outcome, scale):
         _check(lib().qsv_collapse(self.h, q, outcome, scale))
