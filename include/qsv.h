@@ -231,17 +231,23 @@ int qsv_collapse(qsv_state *s, int q, int outcome, double scale);
 /* Measure qubit q with draw r in [0,1] (the value of the reference's
  * randGenerator() call): outcome = ((float)r > p0), then collapse and
  * renormalise by 1/sqrt(p).  Replaces qubitMeasure (CPUQuantumGates.cpp:1071-1126).
- * When (float)r lies within the rounding band of p0, p0 is recomputed in the
- * CPU simulator's serial summation order before deciding. */
+ * When (float)r lies within the rounding band of p0 (2^(n+1) eps), p0 is
+ * recomputed in the CPU simulator's serial summation order (on the host,
+ * with the C library's hypot) before deciding, for states of up to 30 qubits
+ * (the reference's limit, QPandaAPI.cpp:87); larger states decide on the
+ * FP64 device sum. */
 int qsv_measure(qsv_state *s, int q, double r, int *outcome);
 
-/* Full marginal distribution over qubits[0..m-1] (m <= 24) into out[2^m];
- * outcome bit j = bit(qubits[j]) (caller order, CPUQuantumGates.cpp:171-174). */
+/* Full marginal distribution over qubits[0..m-1] (1 <= m <= n, distinct)
+ * into out[2^m]; outcome bit j = bit(qubits[j]) (caller order,
+ * CPUQuantumGates.cpp:171-174).  FP64, deterministic order. */
 int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out);
 
 /* pMeasure compatibility view: the distribution sorted by probability
- * descending, truncated to 10 entries (CPUQuantumGates.cpp:159-190).
- * Writes *count entries of (index, probability). */
+ * descending, truncated to 10 entries (CPUQuantumGates.cpp:159-190), for any
+ * 1 <= m <= n.  The top 10 are selected on the device; equal probabilities
+ * are ordered by index (the reference's std::sort leaves their order
+ * unspecified).  Writes *count = min(10, 2^m) entries of (index, probability). */
 int qsv_pmeasure(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count);
 
 /* sum_{i=0}^{n-2} <Z_i Z_{i+1}> in FP64. */

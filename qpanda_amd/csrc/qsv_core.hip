@@ -376,9 +376,12 @@ int qsv_collapse(qsv_state *s, int q, int outcome, double scale)
     return launch_collapse(s, q, outcome ? 1 : 0, scale);
 }
 
-// Largest state for which a tie is re-decided by the serial-order sum
-// (2^23 sequential terms on one lane).
-static const int kSerialTieMaxQubits = 24;
+// Largest state for which a tie is re-decided by the serial-order sum: the
+// reference runs programs of at most 30 qubits (QPandaAPI.cpp:87), so above
+// that no reference order exists to reproduce and the FP64 tree decides.  At
+// 30 qubits the host sum takes a few seconds (a tie needs |(float)r - p0| <=
+// 2^31 eps, about one measurement in 10^6).
+static const int kSerialTieMaxQubits = 30;
 
 int qsv_measure(qsv_state *s, int q, double r, int *outcome)
 {
@@ -438,11 +441,10 @@ int qsv_sample_shots_copy(qsv_state *s, const int *qubits, int m, int64_t gate_d
     return sample_copy(s, qubits, m, gate_draws, rng, shots, out, ties);
 }
 
-int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out)
+static int check_marginal_args(qsv_state *s, const int *qubits, int m)
 {
-    CHECK_STATE(s);
-    if (!qubits || !out) return fail(QSV_ERR_ARG, "null");
-    if (m < 1 || m > 24 || m > s->n) return fail(QSV_ERR_ARG, "marginal over 1..24 qubits");
+    if (!qubits) return fail(QSV_ERR_ARG, "null");
+    if (m < 1 || m > s->n) return fail(QSV_ERR_ARG, "marginal over 1..n qubits");
     uint64_t seen = 0;
     for (int j = 0; j < m; j++) {
         int rc = check_qubit(s, qubits[j]);
@@ -450,28 +452,25 @@ int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out)
         if (seen >> qubits[j] & 1) return fail(QSV_ERR_QUBIT, "repeated qubit");
         seen |= uint64_t(1) << qubits[j];
     }
+    return QSV_OK;
+}
+
+int qsv_marginals(qsv_state *s, const int *qubits, int m, double *out)
+{
+    CHECK_STATE(s);
+    if (!out) return fail(QSV_ERR_ARG, "null");
+    int rc = check_marginal_args(s, qubits, m);
+    if (rc) return rc;
     return marginals(s, qubits, m, out);
 }
 
 int qsv_pmeasure(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count)
 {
+    CHECK_STATE(s);
     if (!idx || !prob || !count) return fail(QSV_ERR_ARG, "null");
-    std::vector<double> full((size_t)1 << std::max(0, std::min(m, 24)));
-    int rc = qsv_marginals(s, qubits, m, full.data());
+    int rc = check_marginal_args(s, qubits, m);
     if (rc) return rc;
-    std::vector<std::pair<uint64_t, double>> v(full.size());
-    for (size_t i = 0; i < full.size(); i++) v[i] = {i, full[i]};
-    // CPUQuantumGates.cpp:178-187: sort by probability descending, keep 10.
-    std::sort(v.begin(), v.end(), [](const std::pair<uint64_t, double> &a, const std::pair<uint64_t, double> &b) {
-        return a.second > b.second;
-    });
-    const int keep = (int)std::min<size_t>(10, v.size());
-    for (int i = 0; i < keep; i++) {
-        idx[i] = v[i].first;
-        prob[i] = v[i].second;
-    }
-    *count = keep;
-    return QSV_OK;
+    return pmeasure_top(s, qubits, m, idx, prob, count);
 }
 
 // ------------------------------------------------------------------ RNG ----

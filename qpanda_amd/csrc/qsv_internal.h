@@ -100,6 +100,7 @@ int reduce_prob0(qsv_state *s, int q, double *p0);
 int reduce_norm2(qsv_state *s, double *v);
 int reduce_zz_chain(qsv_state *s, double *v);
 int serial_prob0(qsv_state *s, int q, double *p0);
+int serial_prob0_host(qsv_state *s, int q, double *p0);   // qsv_io.cpp
 int inner(qsv_state *a, qsv_state *b, double *re, double *im);
 int copy_state(qsv_state *d, qsv_state *s);
 int sample_tree(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int32_t *rng, int64_t shots,
@@ -109,6 +110,7 @@ int sample_copy(qsv_state *s, const int *qubits, int m, int64_t gate_draws, int3
 int gather(qsv_state *s, const uint64_t *idx, int64_t count, double *out);
 int launch_collapse(qsv_state *s, int q, int outcome, double scale);
 int marginals(qsv_state *s, const int *qubits, int m, double *out);
+int pmeasure_top(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count);
 
 // Fused FP32 MFMA block (qsv_fused.hip).
 int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int split);

@@ -9,6 +9,7 @@
 //           getQState text (CPUQuantumGates.cpp:64-82; .fullstate).
 #include <cerrno>
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -111,6 +112,35 @@ int stream_in(qsv_state *s, const std::function<int(uint64_t, uint64_t, double *
     QSV_HIP(hipStreamSynchronize(s->stream));
     return QSV_OK;
 }
+
+}  // namespace
+
+// qubitMeasure's P(0) in the CPU simulator's own order
+// (CPUQuantumGates.cpp:1074-1083): every index with bit q = 0 in ascending
+// order, p += |a| |a| with |a| = hypot(re, im) from the C library (what
+// std::abs of a std::complex<double> computes there), each operation rounded
+// separately (no contraction into an FMA).  The state streams through the
+// pinned chunks of stream_out, the copy of the next chunk overlapping the sum
+// of this one.  Used only inside the measurement tie band.
+int serial_prob0_host(qsv_state *s, int q, double *p0)
+{
+    double p = 0;
+    const uint64_t bit = uint64_t(1) << q;
+    const int rc = stream_out(s, [&](uint64_t first, uint64_t cnt, const double *rec) {
+#pragma clang fp contract(off)
+        for (uint64_t k = 0; k < cnt; k++) {
+            if ((first + k) & bit) continue;
+            const double h = std::hypot(rec[2 * k], rec[2 * k + 1]);
+            p = p + h * h;
+        }
+        return QSV_OK;
+    });
+    if (rc) return rc;
+    *p0 = p;
+    return QSV_OK;
+}
+
+namespace {
 
 struct File {
     FILE *f = nullptr;

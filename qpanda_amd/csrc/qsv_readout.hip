@@ -7,11 +7,12 @@
 // precisions.  QPanda's CUDA counterpart is a shared-memory tree per block
 // followed by one thread summing the partials serially
 // (GPUGates.cu:597-682); its CPU simulator sums serially in index order
-// (CPUQuantumGates.cpp:1077-1083), which serial_prob0 reproduces for the
-// measurement tie band.
+// (CPUQuantumGates.cpp:1077-1083), which serial_prob0 reproduces on the host
+// (serial_prob0_host, qsv_io.cpp) for the measurement tie band.
 #include "qsv_internal.h"
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <cmath>
 
@@ -81,22 +82,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
     for (int i = threadIdx.x; i < n; i += kBlock) acc += partial[i];
     const double t = block_sum(acc, lds);
     if (threadIdx.x == 0) *out = t;
-}
-
-// CPU-simulator order: for each block of 2*st, for j in the lower half,
-// p += hypot(re, im)^2, every operation rounded separately.
-template <typename T>
-__global__ void k_serial_prob0(const T *__restrict__ a, uint64_t dim, uint64_t st, double *out)
-{
-#pragma clang fp contract(off)
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double p = 0;
-    for (uint64_t i = 0; i < dim; i += 2 * st)
-        for (uint64_t j = i; j < i + st; j++) {
-            const double h = hypot((double)a[j].x, (double)a[j].y);
-            p = p + h * h;
-        }
-    *out = p;
 }
 
 template <typename T>
@@ -175,6 +160,10 @@ __global__ __launch_bounds__(kBlock) void k_copy(TD *__restrict__ d, const TS *_
 
 // ---- marginals --------------------------------------------------------------
 constexpr int kTileBits = 12;
+// Marginals over more qubits use k_marg_direct (the tile path's final pass
+// launches 2^m workgroups of kBlock threads, within the grid limits only for
+// small m).
+constexpr int kMargTileMaxQubits = 16;
 
 // Stage 1: one tile of 2^tb consecutive amplitudes per workgroup; probabilities
 // into LDS, then pairwise sums over every unmeasured tile bit in ascending bit
@@ -237,6 +226,143 @@ __global__ __launch_bounds__(kBlock) void k_marg_final(const double *__restrict_
     }
     const double s = block_sum(acc, lds);
     if (threadIdx.x == 0) out[o] = s;
+}
+
+// Marginals over many qubits (m > 16, up to m = n): one thread per outcome
+// t, whose measured bits (in ascending qubit order) are t's bits; it sums
+// |a|^2 over the 2^(n-m) settings u of the unmeasured qubits in ascending
+// order (u steps through the subsets of the unmeasured mask) and writes the
+// sum at the caller-order index (bit j = bit of qubits[j]).  QPanda's CUDA
+// back end has its own branch for large marginals (GPUGates.cu:2033-2102).
+struct CallerMap {
+    int m;
+    int pos[QSV_MAX_QUBITS];     // k-th smallest measured qubit -> caller bit j
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_marg_direct(const T *__restrict__ a, uint64_t nout, InsertSet unmeas_ins,
+                                                         uint64_t unmeas_mask, CallerMap cm, double *__restrict__ out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nout) return;
+    const uint64_t base = expand_index(t, unmeas_ins);
+    double acc = 0;
+    uint64_t u = 0;
+    do {
+        acc += norm_of(a[base | u]);
+        u = ((u | ~unmeas_mask) + 1) & unmeas_mask;
+    } while (u != 0);
+    uint64_t o = 0;
+    for (int k = 0; k < cm.m; k++) o |= ((t >> k) & 1) << cm.pos[k];
+    out[o] = acc;
+}
+
+// ---- top 10 of a distribution on the device (pMeasure's view) ---------------
+// Order: probability descending, ties by index ascending.  (The reference
+// sorts with std::sort, whose order among equal probabilities is unspecified,
+// CPUQuantumGates.cpp:178-187.)  Each thread keeps a sorted list of its 10
+// best in registers (insertion with constant indices); a workgroup then draws
+// its 10 best by ten rounds of argmax over the threads' list heads.
+constexpr int kTopK = 10;
+
+__device__ __forceinline__ bool topk_better(double a, uint64_t ia, double b, uint64_t ib)
+{
+    return a > b || (a == b && ia < ib);
+}
+
+__device__ __forceinline__ void topk_init(double (&v)[kTopK], uint64_t (&ix)[kTopK])
+{
+#pragma unroll
+    for (int k = 0; k < kTopK; k++) {
+        v[k] = -1.0;                 // below every probability
+        ix[k] = ~uint64_t(0);
+    }
+}
+
+__device__ __forceinline__ void topk_insert(double (&v)[kTopK], uint64_t (&ix)[kTopK], double x, uint64_t xi)
+{
+    if (!topk_better(x, xi, v[kTopK - 1], ix[kTopK - 1])) return;
+    v[kTopK - 1] = x;
+    ix[kTopK - 1] = xi;
+#pragma unroll
+    for (int k = kTopK - 1; k > 0; k--) {
+        if (topk_better(v[k], ix[k], v[k - 1], ix[k - 1])) {
+            const double tv = v[k];
+            v[k] = v[k - 1];
+            v[k - 1] = tv;
+            const uint64_t ti = ix[k];
+            ix[k] = ix[k - 1];
+            ix[k - 1] = ti;
+        }
+    }
+}
+
+// The workgroup's 10 best of all threads' lists into ov / oi (thread 0 writes).
+__device__ __forceinline__ void topk_block(double (&v)[kTopK], uint64_t (&ix)[kTopK], double *ov, uint64_t *oi)
+{
+    __shared__ double sv[kBlock / 64];
+    __shared__ uint64_t si[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = 0; r < kTopK; r++) {
+        double hv = v[0];
+        uint64_t hi = ix[0];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double ov2 = __shfl_xor(hv, off, 64);
+            const uint64_t oi2 = __shfl_xor(hi, off, 64);
+            if (topk_better(ov2, oi2, hv, hi)) {
+                hv = ov2;
+                hi = oi2;
+            }
+        }
+        if (lane == 0) {
+            sv[wave] = hv;
+            si[wave] = hi;
+        }
+        __syncthreads();
+        double bv = sv[0];
+        uint64_t bi = si[0];
+        for (int w = 1; w < kBlock / 64; w++)
+            if (topk_better(sv[w], si[w], bv, bi)) {
+                bv = sv[w];
+                bi = si[w];
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ov[r] = bv;
+            oi[r] = bi;
+        }
+        if (ix[0] == bi && v[0] == bv) {             // this thread's head won: pop it
+#pragma unroll
+            for (int k = 0; k + 1 < kTopK; k++) {
+                v[k] = v[k + 1];
+                ix[k] = ix[k + 1];
+            }
+            v[kTopK - 1] = -1.0;
+            ix[kTopK - 1] = ~uint64_t(0);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_topk_partial(const double *__restrict__ vals, uint64_t count,
+                                                          double *__restrict__ pv, uint64_t *__restrict__ pi)
+{
+    double v[kTopK];
+    uint64_t ix[kTopK];
+    topk_init(v, ix);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) topk_insert(v, ix, vals[i], i);
+    topk_block(v, ix, pv + (uint64_t)blockIdx.x * kTopK, pi + (uint64_t)blockIdx.x * kTopK);
+}
+
+__global__ __launch_bounds__(kBlock) void k_topk_final(const double *__restrict__ pv, const uint64_t *__restrict__ pi,
+                                                        int ncand, double *__restrict__ ov, uint64_t *__restrict__ oi)
+{
+    double v[kTopK];
+    uint64_t ix[kTopK];
+    topk_init(v, ix);
+    for (int i = threadIdx.x; i < ncand; i += kBlock) topk_insert(v, ix, pv[i], pi[i]);
+    topk_block(v, ix, ov, oi);
 }
 
 // Partition of a reduction: nblk <= kRedParts workgroups of `chunk`
@@ -424,18 +550,8 @@ int copy_state(qsv_state *d, qsv_state *s)
 
 int serial_prob0(qsv_state *s, int q, double *p0)
 {
-    double *out = s->red + (kRedSlots - 2);
-    const uint64_t st = uint64_t(1) << q;
-    if (s->prec == QSV_C128)
-        hipLaunchKernelGGL(k_serial_prob0<c128>, dim3(1), dim3(64), 0, s->stream, (const c128 *)s->amps, s->dim, st, out);
-    else
-        hipLaunchKernelGGL(k_serial_prob0<c64>, dim3(1), dim3(64), 0, s->stream, (const c64 *)s->amps, s->dim, st, out);
-    s->launches++;
-    QSV_HIP(hipGetLastError());
-    QSV_HIP(hipMemcpyAsync(s->host_red, out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    QSV_HIP(hipStreamSynchronize(s->stream));
-    *p0 = s->host_red[0];
-    return QSV_OK;
+    QSV_HIP(hipStreamSynchronize(s->stream));        // the gates before the measurement
+    return serial_prob0_host(s, q, p0);
 }
 
 int launch_collapse(qsv_state *s, int q, int outcome, double scale)
@@ -454,8 +570,39 @@ int launch_collapse(qsv_state *s, int q, int outcome, double scale)
     return QSV_OK;
 }
 
-int marginals(qsv_state *s, const int *qubits, int m, double *out)
+// Full marginal distribution over qubits[0..m-1] into device memory: the
+// 2^m doubles at *dout, inside s->aux, followed by `extra` free bytes.
+static int marginals_dev(qsv_state *s, const int *qubits, int m, size_t extra, double **dout)
 {
+    const size_t out_bytes = sizeof(double) * ((size_t)1 << m);
+    if (m > kMargTileMaxQubits) {
+        uint64_t meas = 0;
+        for (int j = 0; j < m; j++) meas |= uint64_t(1) << qubits[j];
+        const uint64_t unmeas = (s->dim - 1) & ~meas;
+        CallerMap cm{};
+        cm.m = m;
+        int k = 0;
+        for (int q = 0; q < s->n; q++)
+            if (meas >> q & 1) {
+                for (int j = 0; j < m; j++)
+                    if (qubits[j] == q) cm.pos[k] = j;
+                k++;
+            }
+        int rc = ensure_aux(s, out_bytes + extra);
+        if (rc) return rc;
+        double *out = (double *)s->aux;
+        const uint64_t nout = uint64_t(1) << m;
+        const InsertSet ins = insert_set_from_mask(unmeas);
+        const dim3 g((unsigned)((nout + kBlock - 1) / kBlock)), t(kBlock);
+        if (s->prec == QSV_C128)
+            hipLaunchKernelGGL(k_marg_direct<c128>, g, t, 0, s->stream, (const c128 *)s->amps, nout, ins, unmeas, cm, out);
+        else
+            hipLaunchKernelGGL(k_marg_direct<c64>, g, t, 0, s->stream, (const c64 *)s->amps, nout, ins, unmeas, cm, out);
+        s->launches++;
+        QSV_HIP(hipGetLastError());
+        *dout = out;
+        return QSV_OK;
+    }
     const int tb = std::min(s->n, kTileBits);
     uint64_t low_mask = 0, high_tile_mask = 0;
     MargMap mm{};
@@ -478,22 +625,66 @@ int marginals(qsv_state *s, const int *qubits, int m, double *out)
     }
     const uint64_t ntiles = s->dim >> tb;
     const size_t partial_bytes = sizeof(double) * (size_t)ntiles * ((size_t)1 << mm.ml);
-    const size_t out_bytes = sizeof(double) * ((size_t)1 << m);
-    int rc = ensure_aux(s, partial_bytes + out_bytes);
+    int rc = ensure_aux(s, partial_bytes + out_bytes + extra);
     if (rc) return rc;
-    double *partial = (double *)s->aux;
-    double *dout = (double *)((char *)s->aux + partial_bytes);
+    double *partial = (double *)((char *)s->aux + out_bytes + extra);
+    double *out = (double *)s->aux;
     if (s->prec == QSV_C128)
         hipLaunchKernelGGL(k_marg_tile<c128>, dim3((unsigned)ntiles), dim3(kBlock), 0, s->stream, (const c128 *)s->amps, tb, low_mask, mm.ml, low_pos, partial);
     else
         hipLaunchKernelGGL(k_marg_tile<c64>, dim3((unsigned)ntiles), dim3(kBlock), 0, s->stream, (const c64 *)s->amps, tb, low_mask, mm.ml, low_pos, partial);
     InsertSet high_ins = insert_set_from_mask(high_tile_mask);
     const uint64_t each = ntiles >> high_ins.n;
-    hipLaunchKernelGGL(k_marg_final, dim3((unsigned)(uint64_t(1) << m)), dim3(kBlock), 0, s->stream, partial, mm, high_ins, each, dout);
+    hipLaunchKernelGGL(k_marg_final, dim3((unsigned)(uint64_t(1) << m)), dim3(kBlock), 0, s->stream, partial, mm, high_ins, each, out);
     s->launches += 2;
     QSV_HIP(hipGetLastError());
-    QSV_HIP(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, s->stream));
+    *dout = out;
+    return QSV_OK;
+}
+
+int marginals(qsv_state *s, const int *qubits, int m, double *out)
+{
+    double *dout = nullptr;
+    int rc = marginals_dev(s, qubits, m, 0, &dout);
+    if (rc) return rc;
+    QSV_HIP(hipMemcpyAsync(out, dout, sizeof(double) * ((size_t)1 << m), hipMemcpyDeviceToHost, s->stream));
     QSV_HIP(hipStreamSynchronize(s->stream));
+    return QSV_OK;
+}
+
+// pMeasure's view (CPUQuantumGates.cpp:159-190): the 10 most probable outcomes
+// of the marginal distribution, found on the device (no host copy or sort of
+// the 2^m probabilities).
+int pmeasure_top(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count)
+{
+    const uint64_t nv = uint64_t(1) << m;
+    const unsigned nblk = (unsigned)std::min<uint64_t>(1024, (nv + kBlock - 1) / kBlock);
+    const size_t cand = (size_t)nblk * kTopK;
+    // after the distribution: partial candidates (values, indices), then the result
+    const size_t extra = cand * 16 + kTopK * 16 + 256;
+    double *dv = nullptr;
+    int rc = marginals_dev(s, qubits, m, extra, &dv);
+    if (rc) return rc;
+    char *x = (char *)dv + sizeof(double) * nv;
+    x = (char *)(((uintptr_t)x + 255) & ~(uintptr_t)255);
+    double *pv = (double *)x;
+    uint64_t *pi = (uint64_t *)(pv + cand);
+    double *ov = (double *)(pi + cand);
+    uint64_t *oi = (uint64_t *)(ov + kTopK);
+    hipLaunchKernelGGL(k_topk_partial, dim3(nblk), dim3(kBlock), 0, s->stream, dv, nv, pv, pi);
+    hipLaunchKernelGGL(k_topk_final, dim3(1), dim3(kBlock), 0, s->stream, pv, pi, (int)cand, ov, oi);
+    s->launches += 2;
+    QSV_HIP(hipGetLastError());
+    static_assert(2 * kTopK <= 64, "pinned mirror holds the result");
+    QSV_HIP(hipMemcpyAsync(s->host_red, ov, kTopK * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    QSV_HIP(hipMemcpyAsync(s->host_red + kTopK, oi, kTopK * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+    QSV_HIP(hipStreamSynchronize(s->stream));
+    const int keep = (int)std::min<uint64_t>(kTopK, nv);
+    for (int i = 0; i < keep; i++) {
+        prob[i] = s->host_red[i];
+        std::memcpy(&idx[i], &s->host_red[kTopK + i], sizeof(uint64_t));
+    }
+    *count = keep;
     return QSV_OK;
 }
 

@@ -137,11 +137,12 @@ __global__ void k_tree_level(const double *__restrict__ child, double *__restric
 
 __device__ __forceinline__ double tie_band(int n, float fr)
 {
-    // the measurement's own band (qsv_measure) for n <= 24; above that the
-    // per-shot path has no serial re-decision, so the band only covers the
+    // the measurement's own band (qsv_measure) for n <= 30; above that the
+    // per-shot path has no serial re-decision (no reference order exists
+    // beyond the reference's 30 qubits), so the band only covers the
     // rounding difference between the two evaluations
     const double f = fmax(1.0, (double)fr);
-    return n <= 24 ? ldexp(DBL_EPSILON, n + 1) * f : 1e-12 * f;
+    return n <= 30 ? ldexp(DBL_EPSILON, n + 1) * f : 1e-12 * f;
 }
 
 // one thread per shot over the top T levels; tree level j lives at offset 2^j - 1
