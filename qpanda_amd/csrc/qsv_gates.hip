@@ -119,31 +119,33 @@ __global__ __launch_bounds__(kBlock) void k_block_f64(T *__restrict__ a, uint64_
     const int d = ba.d;
     for (int i = threadIdx.x; i < d * d * 2; i += kBlock) su[i] = u[i];
     __syncthreads();
-    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= groups) return;
-    const uint64_t b = expand_index(g, ins) | ba.ctrl;
-    double xr[32], xi[32];
-#pragma unroll
-    for (int c = 0; c < 32; c++) {
-        if (c < d) {
-            T v = a[b + ba.off[c]];
-            xr[c] = v.x; xi[c] = v.y;
-        }
-    }
-    for (int r = 0; r < d; r++) {
-        double sr = 0, si = 0;
+    // grid-stride: a launch stays below 2^32 threads at any state size
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += stride) {
+        const uint64_t b = expand_index(g, ins) | ba.ctrl;
+        double xr[32], xi[32];
 #pragma unroll
         for (int c = 0; c < 32; c++) {
             if (c < d) {
-                const double ur = su[2 * (r * d + c)], ui = su[2 * (r * d + c) + 1];
-                sr += ur * xr[c] - ui * xi[c];
-                si += ur * xi[c] + ui * xr[c];
+                T v = a[b + ba.off[c]];
+                xr[c] = v.x; xi[c] = v.y;
             }
         }
-        T y;
-        y.x = (typename Real<T>::type)sr;
-        y.y = (typename Real<T>::type)si;
-        a[b + ba.off[r]] = y;
+        for (int r = 0; r < d; r++) {
+            double sr = 0, si = 0;
+#pragma unroll
+            for (int c = 0; c < 32; c++) {
+                if (c < d) {
+                    const double ur = su[2 * (r * d + c)], ui = su[2 * (r * d + c) + 1];
+                    sr += ur * xr[c] - ui * xi[c];
+                    si += ur * xi[c] + ui * xr[c];
+                }
+            }
+            T y;
+            y.x = (typename Real<T>::type)sr;
+            y.y = (typename Real<T>::type)si;
+            a[b + ba.off[r]] = y;
+        }
     }
 }
 
@@ -158,6 +160,10 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(T *__restrict__ a, uint64_
         a[i] = v;
     }
 }
+
+// Workgroups of a grid-stride launch: 2^20 x 256 threads, below HIP's 2^32
+// threads per launch.
+constexpr unsigned kMaxGrid = 1u << 20;
 
 inline unsigned grid_for(uint64_t items, uint64_t per_block)
 {
@@ -240,7 +246,7 @@ int launch_block_dev(qsv_state *s, const int *qubits, int k, const double *u_dev
     }
     InsertSet ins = insert_set_from_mask(mask);
     const uint64_t groups = s->dim >> ins.n;
-    const unsigned grid = grid_for(groups, kBlock);
+    const unsigned grid = std::min(grid_for(groups, kBlock), kMaxGrid);
     if (s->prec == QSV_C128)
         hipLaunchKernelGGL(k_block_f64<c128>, dim3(grid), dim3(kBlock), 0, s->stream, (c128 *)s->amps, groups, ins, ba, u_dev);
     else

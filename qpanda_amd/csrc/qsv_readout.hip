@@ -88,17 +88,19 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_collapse(T *__restrict__ a, uint64_t npairs, InsertSet ins,
                                                       uint64_t bit, int outcome, double scale)
 {
-    const uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npairs) return;
-    const uint64_t i0 = expand_index(p, ins), i1 = i0 | bit;
-    const uint64_t keep = outcome ? i1 : i0, drop = outcome ? i0 : i1;
-    using R = decltype(T{}.x);
-    T v = a[keep], z;
-    z.x = 0; z.y = 0;
-    v.x = (R)((double)v.x * scale);
-    v.y = (R)((double)v.y * scale);
-    a[keep] = v;
-    a[drop] = z;
+    // grid-stride: a launch stays below 2^32 threads at any state size
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npairs; p += stride) {
+        const uint64_t i0 = expand_index(p, ins), i1 = i0 | bit;
+        const uint64_t keep = outcome ? i1 : i0, drop = outcome ? i0 : i1;
+        using R = decltype(T{}.x);
+        T v = a[keep], z;
+        z.x = 0; z.y = 0;
+        v.x = (R)((double)v.x * scale);
+        v.y = (R)((double)v.y * scale);
+        a[keep] = v;
+        a[drop] = z;
+    }
 }
 
 // <a|b> over two states of possibly different precisions: per-block partial
@@ -560,7 +562,7 @@ int launch_collapse(qsv_state *s, int q, int outcome, double scale)
     ins.n = 1;
     ins.pos[0] = q;
     const uint64_t npairs = s->dim >> 1;
-    const unsigned grid = (unsigned)((npairs + kBlock - 1) / kBlock);
+    const unsigned grid = (unsigned)std::min<uint64_t>((npairs + kBlock - 1) / kBlock, uint64_t(1) << 20);
     if (s->prec == QSV_C128)
         hipLaunchKernelGGL(k_collapse<c128>, dim3(grid), dim3(kBlock), 0, s->stream, (c128 *)s->amps, npairs, ins, uint64_t(1) << q, outcome, scale);
     else
