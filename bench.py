@@ -154,25 +154,67 @@ def measured_traffic(kernel_class, a, headline, k):
         return None
 
 
+def physical_cores() -> int:
+    """Physical cores among the CPUs this process may run on (distinct
+    package/core pairs of its affinity set)."""
+    try:
+        cores = set()
+        for c in os.sched_getaffinity(0):
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        return max(1, len(cores))
+    except (OSError, AttributeError):
+        return os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Threads for the OpenMP baseline: the physical cores this process may use,
+    capped by OMP_NUM_THREADS when the environment sets it (the share of the
+    host a job is granted)."""
+    phys = physical_cores()
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(env, phys) if env > 0 else phys), phys
+
+
 def cpu_baseline(circ, seconds, stride):
     """QPanda's CPU simulator (reference build) on records 0, stride, 2*stride, ...
-    of the benchmark circuit; falls back to the oracle port when the reference
-    build is absent."""
+    of the benchmark circuit: the OpenMP build on the physical cores (threads
+    bound close), the as-shipped 1-thread build, and a check that the OpenMP
+    build's state equals the 1-thread build's (its CNOT / Toffoli loops race,
+    CPUQuantumGates.cpp:678/688, :1391/1414).  Falls back to the oracle port
+    when the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    from qpanda_amd import circuits as Cc
+    threads, phys = cpu_threads()
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "bench.qsvc")
         circ.save(path)
         if O.ref_available():
             r = O.ref_time(path, stride, seconds, omp=True, threads=threads)
+            r1 = O.ref_time(path, stride, max(8.0, seconds / 2), omp=False)
+            # the same gate distribution at 22 qubits, first 300 records, both builds
+            chk = os.path.join(td, "check.qsvc")
+            Cc.random_layers(22, 20, 2222, haar_frac=0.1).save(chk)
+            d_omp = O.ref_state_digest(chk, 300, omp=True, threads=threads)
+            d_one = O.ref_state_digest(chk, 300, omp=False)
             return {"value": r["gates"] / r["seconds"], "unit": "gate-apps/s", "cores": threads,
                     "kind": "reference",
-                    "sample": (f"QPanda CPUQuantumGates (OpenMP build, OMP_NUM_THREADS={threads}) at "
-                               f"n={circ.n}, c128: records 0,{stride},{2 * stride},... of the benchmark "
-                               f"circuit, {r['gates']} gates ({r['reference_calls']} reference calls; CZ as "
-                               f"CR(pi), Haar pairs as an 18-call KAK-shaped template) in "
-                               f"{r['seconds']:.1f} s, state allocation excluded")}
+                    "sample": (f"QPanda CPUQuantumGates (OpenMP build, OMP_NUM_THREADS={threads}, "
+                               f"OMP_PROC_BIND=close) at n={circ.n}, c128: records 0,{stride},{2 * stride},... "
+                               f"of the benchmark circuit, {r['gates']} gates ({r['reference_calls']} reference "
+                               f"calls; CZ as CR(pi), Haar pairs as an 18-call KAK-shaped template) in "
+                               f"{r['seconds']:.1f} s, state allocation excluded"),
+                    "physical_cores_visible": phys,
+                    "one_thread_as_shipped": {
+                        "value": r1["gates"] / r1["seconds"], "unit": "gate-apps/s", "gates": r1["gates"],
+                        "seconds": r1["seconds"],
+                        "build": "no OpenMP, -O3 (QPandaSDK/CMakeLists.txt:65-71)"},
+                    "omp_state_equals_one_thread": d_omp == d_one,
+                    "omp_check_sample": ("n=22, first 300 records of the benchmark gate distribution "
+                                         f"(seed 2222), FNV-1a of the state bytes: OpenMP {d_omp}, "
+                                         f"1 thread {d_one}")}
         # port: the oracle restatement, single thread
         from qpanda_amd.circuits import Circuit
         st = O.State(circ.n)
@@ -656,7 +698,7 @@ def cpu_baseline_batch(nq, ang, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from qpanda_amd import engine as E
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads, _ = cpu_threads()
     if not O.ref_available():
         return None
     done, spent = 0, 0.0
@@ -670,7 +712,7 @@ def cpu_baseline_batch(nq, ang, seconds):
             if spent > seconds:
                 break
     return {"value": done / spent, "unit": "instances/s", "cores": threads, "kind": "reference",
-            "sample": f"QPanda CPUQuantumGates (OpenMP build, OMP_NUM_THREADS={threads}), c128, the first "
+            "sample": f"QPanda CPUQuantumGates (OpenMP build, OMP_NUM_THREADS={threads}, OMP_PROC_BIND=close), c128, the first "
                       f"{done} instances of the batch one after another in {spent:.1f} s (allocation excluded)"}
 
 

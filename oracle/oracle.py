@@ -270,17 +270,36 @@ def ref_run(circ_path: str, out_prefix: str, seed: int, shots: int, omp: bool = 
     subprocess.check_call([exe, "run", circ_path, out_prefix, str(seed), str(shots)])
 
 
-def ref_time(circ_path: str, stride: int, max_seconds: float, omp: bool = True,
-             threads: int | None = None) -> dict:
-    """Time QPanda's own CPU simulator on records 0, stride, 2*stride, ... of a
-    circuit file until max_seconds has passed (allocation excluded)."""
-    import json
-    exe = os.path.join(REF_DIR, "qpanda_ref_omp" if omp else "qpanda_ref")
+def _ref_env(threads):
     env = dict(os.environ)
     if threads is not None:
         env["OMP_NUM_THREADS"] = str(threads)
-    out = subprocess.check_output([exe, "time", circ_path, str(stride), str(max_seconds)], env=env)
+        env["OMP_PROC_BIND"] = "close"
+    return env
+
+
+def ref_time(circ_path: str, stride: int, max_seconds: float, omp: bool = True,
+             threads: int | None = None) -> dict:
+    """Time QPanda's own CPU simulator on records 0, stride, 2*stride, ... of a
+    circuit file until max_seconds has passed (allocation excluded).  omp:
+    the OpenMP build with `threads` threads bound close; else the as-shipped
+    build (no OpenMP, QPandaSDK/CMakeLists.txt:65-71), one thread."""
+    import json
+    exe = os.path.join(REF_DIR, "qpanda_ref_omp" if omp else "qpanda_ref")
+    out = subprocess.check_output([exe, "time", circ_path, str(stride), str(max_seconds)],
+                                  env=_ref_env(threads if omp else None))
     return json.loads(out.decode().strip().splitlines()[-1])
+
+
+def ref_state_digest(circ_path: str, records: int, omp: bool, threads: int | None = None) -> str:
+    """FNV-1a digest of the reference simulator's state after the first
+    `records` records: equal digests mean bit-identical states, which checks
+    the OpenMP build's racy CNOT / Toffoli loops (CPUQuantumGates.cpp:678/688,
+    :1391/1414) against the 1-thread build."""
+    import json
+    exe = os.path.join(REF_DIR, "qpanda_ref_omp" if omp else "qpanda_ref")
+    out = subprocess.check_output([exe, "check", circ_path, str(records)], env=_ref_env(threads if omp else None))
+    return json.loads(out.decode().strip().splitlines()[-1])["fnv1a"]
 
 
 # ---- QRunes programs: executor and run loop -------------------------------

@@ -22,6 +22,9 @@
 //   time CIRCUIT STRIDE MAX_SECONDS     -> one JSON line: wall time of records 0, STRIDE,
 //                                          2*STRIDE, ... until MAX_SECONDS has passed
 //                                          (CZ as CR(pi); U2 as a 3-CNOT template)
+//   check CIRCUIT NRECORDS              -> one JSON line: FNV-1a digest of the state bytes
+//                                          after records 0 .. NRECORDS-1 (same substitutions),
+//                                          to compare an OpenMP build with the 1-thread one
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -187,10 +190,42 @@ static int cmd_time(int argc, char **argv)
     return 0;
 }
 
+static int cmd_check(int argc, char **argv)
+{
+    if (argc < 4) return 2;
+    Circuit c;
+    if (!load_circuit(argv[2], c)) { std::fprintf(stderr, "cannot read %s\n", argv[2]); return 1; }
+    const long nrec = std::atol(argv[3]);
+    std::memset(&g_tm, 0, sizeof g_tm);
+    g_tm.tm_year = 1 - 70;
+    CPUQuantumGates g;
+    QuantumGateParam param;
+    param.setQBitNum(c.n);
+    if (g.initState(&param) != qErrorNone) return 1;
+    std::map<size_t, int> creg;
+    long done = 0;
+    for (size_t i = 0; i < c.gates.size() && done < nrec; i++, done++) {
+        const qsv_gate &r = c.gates[i];
+        if (r.kind == QSV_GATE_U2) apply_u2_template(g, r.q0, r.q1);
+        else if (r.kind == QSV_GATE_U1) { g.RZ(r.q0, 0.1, 0, 0); g.RY(r.q0, 0.2, 0, 0); g.RZ(r.q0, 0.3, 0, 0); }
+        else if (!apply(g, r, creg)) return 1;
+    }
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(g.mvQuantumStat.data());
+    const size_t nb = g.mvQuantumStat.size() * 16;
+    for (size_t i = 0; i < nb; i++) h = (h ^ b[i]) * 1099511628211ull;
+    const char *omp = std::getenv("OMP_NUM_THREADS");
+    std::printf("{\"n\": %d, \"records\": %ld, \"fnv1a\": \"%016llx\", \"omp_num_threads\": \"%s\"}\n", c.n, done,
+                (unsigned long long)h, omp ? omp : "");
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc >= 2 && std::strcmp(argv[1], "check") == 0) return cmd_check(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "run") == 0) return cmd_run(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "time") == 0) return cmd_time(argc, argv);
-    std::fprintf(stderr, "usage: %s run CIRCUIT OUTPREFIX SEED SHOTS | time CIRCUIT STRIDE MAX_SECONDS\n", argv[0]);
+    std::fprintf(stderr, "usage: %s run CIRCUIT OUTPREFIX SEED SHOTS | time CIRCUIT STRIDE MAX_SECONDS | "
+                         "check CIRCUIT NRECORDS\n", argv[0]);
     return 2;
 }
