@@ -378,7 +378,9 @@ int qsv_fuse_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int
 
 /* Window plan of a gate list for an n-qubit state (host only, no device
  * needed): the blocks (<= max_k = 4 or 5 qubits) of one sweep each lie in a
- * window of tb qubits that holds qubits 0..min(3, n-1); outputs as
+ * window of tb qubits that holds qubits 0..min(3, n-1) (tb = 7..13, or
+ * 4..13 when tb >= n: a window needs room for a 3-qubit record beside qubits
+ * 0..3); outputs as
  * qsv_fuse_plan plus window_out[b], the window of block b (a MEASURE/RESET
  * record has a window of its own).  No reference counterpart. */
 int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,

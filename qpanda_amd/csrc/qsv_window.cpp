@@ -360,6 +360,10 @@ extern "C" int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const doub
 {
     if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
     if (tb < 4 || tb > kMaxPassTileBits) return fail(QSV_ERR_ARG, "window tile bits must be 4..13");
+    // a window holds qubits 0..3 and grows from there, so it needs room for
+    // the widest record (3 qubits: Toffoli, Fredkin) beside them, unless it
+    // holds the whole state
+    if (tb < 7 && tb < n) return fail(QSV_ERR_ARG, "window tile bits must be >= 7 below the state size");
     std::vector<PlanItem> plan;
     std::vector<int> window;
     int rc = plan_window(gates, ngates, mats, max_k, exact, tb, n, plan, window);

@@ -89,7 +89,7 @@ def apply_window_plan(plan, n, psi, exact=False):
 
 
 @pytest.mark.parametrize("k", [4, 5])
-@pytest.mark.parametrize("tb", [6, 8, 13])
+@pytest.mark.parametrize("tb", [7, 8, 13])
 def test_window_plan_reproduces_random_circuit(k, tb):
     n = 11
     circ = C.random_layers(n, 10, 300 + k + tb, haar_frac=0.2)
@@ -197,3 +197,25 @@ def test_window_blocks_are_mostly_full_width():
     widths = [len(qs) for qs, *_ in E.window_plan(circ, 4, 13) if qs is not None]
     assert min(widths) >= 2
     assert sum(w == 4 for w in widths) >= 0.6 * len(widths), np.bincount(widths)
+
+
+def test_window_plan_tile_bits_range():
+    """A window holds qubits 0..3 and grows from there: below 7 tile bits a
+    3-qubit record may not fit beside them, so tb < 7 is rejected unless the
+    window holds the whole state; from 7 up every record fits and every gate
+    is planned (a Toffoli on qubits 5, 6, 7 included)."""
+    n = 11
+    b = C._Builder(n)
+    for g in C.random_layers(n, 6, 31).gates:
+        b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    b.add(C.TOFFOLI, 5, 6, 7)
+    b.add(C.CSWAP, 10, 4, 8)
+    circ = b.build()
+    for tb in (4, 5, 6):
+        with pytest.raises(E.QsvError):
+            E.window_plan(circ, 4, tb)
+    for tb in (7, 8, 11, 13):
+        plan = E.window_plan(circ, 4, tb)
+        assert sum(ng for _, _, ng, _ in plan) == circ.ngates
+    small = C.random_layers(5, 4, 3)
+    assert sum(ng for _, _, ng, _ in E.window_plan(small, 4, 5)) == small.ngates
