@@ -225,9 +225,16 @@ int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
         }
         int qs[3];
         std::vector<cd> gm;
-        const int nq = gate_matrix(r, mats, exact, qs, gm);
+        int nq = gate_matrix(r, mats, exact, qs, gm);
         if (nq < 0) return fail(QSV_ERR_UNSUPPORTED, "gate kind cannot be fused");
-        if (nq == 0) continue;                       // no-op record (e.g. CNOT(q, q))
+        if (nq == 0) {
+            // a no-op record (CNOT(q, q) and the like) changes no amplitude but
+            // the reference executor still draws once for it: plan it as the
+            // identity on q0 so that its block counts it
+            qs[0] = r.q0;
+            nq = 1;
+            gm = {1, 0, 0, 1};
+        }
         // candidate blocks: open blocks on the gate's qubits
         std::vector<int> cand;
         for (int j = 0; j < nq; j++)

@@ -306,16 +306,11 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
             if (!shape_only || blocks_prog.size() < blocks.size())
                 rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, shape_only);
             if (rc) return rc;
-            // no-op records fuse to nothing; keep them counted (one draw each)
+            // every record is counted by some block (no-op records as the
+            // identity on their qubit), one draw each
             int64_t counted = 0;
             for (const PlanItem &b : blocks) counted += b.ngates;
-            if (blocks.empty()) {
-                PlanItem id;                       // only no-op records: an identity block
-                id.qubits = {0};
-                id.u = {1, 0, 0, 0, 0, 0, 1, 0};
-                blocks.push_back(std::move(id));
-            }
-            blocks.back().ngates += (int64_t)sub.size() - counted;
+            if (counted != (int64_t)sub.size()) return fail(QSV_ERR_ARG, "internal: window records not all counted");
             for (PlanItem &b : blocks) {
                 plan.push_back(std::move(b));
                 window.push_back(wid);

@@ -192,20 +192,32 @@ def test_multi_block_passes_vs_oracle(group, pass_tb, k, split):
     assert np.abs(got - ref.psi).max() < 1e-5
 
 
-def test_passes_with_measurements_keep_rng_accounting():
+@pytest.mark.parametrize("sched", [0, 2, 3])
+def test_passes_with_measurements_keep_rng_accounting(sched):
+    """One draw per record on every fused planner, no-op records (CNOT(q, q),
+    which the reference loop skips but still draws for) included, also where
+    a no-op is alone between two measurements."""
     n = 14
     circ = C.random_layers(n, 6, 5)
     b = C._Builder(n)
     for g in circ.gates:
         b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    b.add(C.CNOT, 4, 4)
     b.add(C.MEASURE, 2, 0)
+    b.add(C.CNOT, 9, 9)
+    b.add(C.MEASURE, 6, 1)
     for g in C.random_layers(n, 3, 6).gates:
         b.add(int(g["kind"]), int(g["q0"]), int(g["q1"]), theta=float(g["theta"]))
+    b.add(C.CNOT, 1, 1)
     c2 = b.build()
+    out_f = np.zeros(64, np.int32)
+    out_p = np.zeros(64, np.int32)
     with E.StateVector(n, E.C64) as f, E.StateVector(n, E.C128) as p:
-        f.run(c2, fuse_k=4, scheduler=2, group=8, rng_state=99)
-        p.run(c2, rng_state=99)
+        f.run(c2, fuse_k=4, split=3, scheduler=sched, group=8, rng_state=99, outcomes=out_f)
+        p.run(c2, rng_state=99, outcomes=out_p)
         assert f.last_rng == p.last_rng
+        if np.array_equal(out_f[:2], out_p[:2]):
+            assert f.fidelity(p) >= 1 - 1e-6
 
 
 @pytest.mark.parametrize("pass_tb", [0, 12])
