@@ -335,6 +335,40 @@ void widen_to(PlanItem &it, int kq)
     widen_with(it, add);
 }
 
+// Make qubit q (if the block holds it) the block's row bit 0, permuting the
+// matrix's rows and columns to match.  The sweep kernels move a block's data
+// as pairs of amplitudes that differ in qubit 0, along rows when qubit 0 is
+// row bit 0.
+void move_qubit_first(PlanItem &it, int q)
+{
+    const int k = (int)it.qubits.size();
+    int j = -1;
+    for (int i = 0; i < k; i++)
+        if (it.qubits[i] == q) j = i;
+    if (j <= 0) return;
+    std::vector<int> order{j};                    // new row bit b <- old row bit order[b]
+    for (int i = 0; i < k; i++)
+        if (i != j) order.push_back(i);
+    const int d = 1 << k;
+    auto old_index = [&](int x) {
+        int o = 0;
+        for (int b = 0; b < k; b++)
+            if (x >> b & 1) o |= 1 << order[b];
+        return o;
+    };
+    std::vector<double> u(it.u.size());
+    for (int r = 0; r < d; r++)
+        for (int c = 0; c < d; c++) {
+            const size_t src = (size_t)old_index(r) * d + old_index(c), dst = (size_t)r * d + c;
+            u[2 * dst] = it.u[2 * src];
+            u[2 * dst + 1] = it.u[2 * src + 1];
+        }
+    std::vector<int> qs(k);
+    for (int b = 0; b < k; b++) qs[b] = it.qubits[order[b]];
+    it.qubits.swap(qs);
+    it.u.swap(u);
+}
+
 // Widen a block with identity on the qubits `add` (not in the block).
 void widen_with(PlanItem &it, const std::vector<int> &add)
 {
@@ -497,6 +531,7 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
         const int kq = std::max(kq_min, (int)it.qubits.size());
         if (s->n < kq + 6) continue;
         widen_to(it, kq_min);
+        move_qubit_first(it, 0);
         mfma[i] = 1;
     }
     std::vector<Launch> launches;

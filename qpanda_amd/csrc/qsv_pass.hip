@@ -58,10 +58,14 @@ __host__ __device__ inline int lds_swizzle(int e)
     return e ^ s;
 }
 
+// How a k = 4 block moves its tile data (see pass_apply4_rows / _cols).
+enum PairMode { PAIR_NONE = 0, PAIR_ROWS = 1, PAIR_COLS = 2 };
+
 struct PassBlock {
     const float *mops;      // operand-ready matrix (layout of fused_operands for the product)
     int kq;                 // 4 or 5
     int ncolbits;           // TB - kq
+    int pair;               // PairMode
     int erow[32];           // swizzled LDS index of row (amplitude) r
     int colp[10];           // swizzled LDS index of the i-th column bit alone
 };
@@ -344,6 +348,130 @@ __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, con
     }
 }
 
+// Paired forms of pass_apply4 for the BF16x6 / FP16x3 products.  Qubit 0 is
+// tile position 0 in every pass, and the LDS swizzle keeps bit 0, so the two
+// amplitudes that differ in qubit 0 sit in adjacent floats of each plane.
+// Every block then moves its data as 8-byte pairs: half the LDS instructions
+// and half the address arithmetic of the dword form (a wave's ds_read_b32
+// moves ~60 B/clk per CU on gfx950, ds_read_b64 ~110).
+//
+// PAIR_ROWS: qubit 0 is the block's row bit 0 (the planner puts it first).
+// A lane reads its column's rows (2t, 2t + 1) with one 8-byte read and writes
+// its output rows (2u, 2u + 1) of both planes with one ds_write2st64_b64.
+template <int TB, int MODE, int NW>
+__device__ __forceinline__ void pass_apply4_rows(float *pre, const PassBlock &pb, const Ops4<MODE> &o, int lane,
+                                                 int wave, float one)
+{
+    static_assert(MODE == 2 || MODE == 3, "paired forms serve the half-precision products");
+    constexpr int NCB = TB - 4;
+    constexpr int PER = ((1 << NCB) >> 5) / NW;
+    static_assert(PER >= 1, "fewer 32-column chunks than waves");
+    constexpr int PLANE = 1 << TB;
+    const int hf = lane >> 5;
+    int elane = hf << (TB + 2);
+#pragma unroll
+    for (int i = 0; i < 5; i++) elane ^= (-((lane >> i) & 1)) & (4 * pb.colp[i]);
+    // output rows 4 j + 2 hf (+1): offset of row 4 j, XOR-ed in the upper half
+    // with the difference to row 4 j + 2
+    const int hfmask = -hf;
+    int wa0[4], wdm[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        wa0[j] = 4 * pb.erow[4 * j];
+        wdm[j] = (wa0[j] ^ (4 * pb.erow[4 * j + 2])) & hfmask;
+    }
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    char *const base = reinterpret_cast<char *>(pre);
+    for (int c = 0; c < PER; c++) {
+        const int chunk = wv + NW * c;
+        int ehi = 0;
+#pragma unroll
+        for (int i = 5; i < NCB; i++)
+            if (chunk >> (i - 5) & 1) ehi ^= 4 * pb.colp[i];
+        const int ecol = elane ^ ehi;
+        float bv[16];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const float2 v = *reinterpret_cast<const float2 *>(base + ((4 * pb.erow[2 * t]) ^ ecol));
+            bv[2 * t] = v.x;
+            bv[2 * t + 1] = v.y;
+        }
+        floatx16 acc[1];
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[0][v] = 0.0f;
+        if constexpr (MODE == 2) pass_bf16x6<1, 2>(acc, o.p, bv);
+        else fp16x3_product<1, 2>(acc, o.h, bv, one);
+        // acc[4 j + u]: row 4 j + 2 hf + (u >> 1), plane u & 1
+        const int wcol = ecol & ~(1 << (TB + 2));
+        // 8-byte stores of both planes (ds_write2st64_b64; measured faster than
+        // dword stores that avoid the register moves the pairs need)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            char *const a = base + (wa0[j] ^ wdm[j] ^ wcol);
+            *reinterpret_cast<float2 *>(a) = make_float2(acc[0][4 * j], acc[0][4 * j + 2]);
+            *reinterpret_cast<float2 *>(a + 4 * PLANE) = make_float2(acc[0][4 * j + 1], acc[0][4 * j + 3]);
+        }
+    }
+}
+
+// PAIR_COLS: qubit 0 is a column of the block (column bit 0).  A wave's two
+// 32-column chunks are the two halves of 64 columns that differ in that bit,
+// so one 8-byte read gives a lane the same row of both of its columns; lane
+// bits take column bits 1..5 and the wave index the bits above.  The two
+// products are independent chains on the same operands.
+template <int TB, int MODE, int NW>
+__device__ __forceinline__ void pass_apply4_cols(float *pre, const PassBlock &pb, const Ops4<MODE> &o, int lane,
+                                                 int wave, float one)
+{
+    static_assert(MODE == 2 || MODE == 3, "paired forms serve the half-precision products");
+    constexpr int NCB = TB - 4;
+    static_assert((1 << NCB) == 64 * NW, "one pair of 32-column chunks per wave");
+    constexpr int PLANE = 1 << TB;
+    const int hf = lane >> 5;
+    int elane = hf << (TB + 2);
+#pragma unroll
+    for (int i = 0; i < 5; i++) elane ^= (-((lane >> i) & 1)) & (4 * pb.colp[i + 1]);
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    int ehi = 0;
+#pragma unroll
+    for (int i = 6; i < NCB; i++)
+        if (wv >> (i - 6) & 1) ehi ^= 4 * pb.colp[i];
+    const int ecol = elane ^ ehi;
+    char *const base = reinterpret_cast<char *>(pre);
+    float bv0[16], bv1[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const float2 v = *reinterpret_cast<const float2 *>(base + ((4 * pb.erow[s]) ^ ecol));
+        bv0[s] = v.x;
+        bv1[s] = v.y;
+    }
+    floatx16 acc0[1], acc1[1];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        acc0[0][v] = 0.0f;
+        acc1[0][v] = 0.0f;
+    }
+    if constexpr (MODE == 2) {
+        pass_bf16x6<1, 2>(acc0, o.p, bv0);
+        pass_bf16x6<1, 2>(acc1, o.p, bv1);
+    } else {
+        fp16x3_product<1, 2>(acc0, o.h, bv0, one);
+        fp16x3_product<1, 2>(acc1, o.h, bv1, one);
+    }
+    // output register v: row 4 (v >> 2) + 2 hf + ((v & 3) >> 1), plane v & 1
+    const int wcol = ecol & ~(1 << (TB + 2));
+    const int hfmask = -hf;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int r0 = 4 * (j >> 1) + (j & 1);
+        const int a0 = 4 * pb.erow[r0], d = (a0 ^ (4 * pb.erow[r0 + 2])) & hfmask;
+        char *const a = base + (a0 ^ d ^ wcol);
+        const int v = 4 * (j >> 1) + 2 * (j & 1);
+        *reinterpret_cast<float2 *>(a) = make_float2(acc0[0][v], acc1[0][v]);
+        *reinterpret_cast<float2 *>(a + 4 * PLANE) = make_float2(acc0[0][v + 1], acc1[0][v + 1]);
+    }
+}
+
 // ANY5: some block of the pass has 5 qubits (otherwise only the k = 4 product
 // is compiled in, which keeps the register count at the k = 4 level).
 template <int TB, int MODE, bool ANY5>
@@ -383,7 +511,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
         *reinterpret_cast<float2 *>(pim + e) = make_float2(buf[i].y * ls, buf[i].w * ls);
     }
     __syncthreads();
-    if (ANY5) {
+    if constexpr (ANY5) {
         for (int g = 0; g < pa.nblocks; g++) {
             if (pa.b[g].kq == 4) pass_apply<4, MODE, NW>(pre, pim, pa.b[g], lane, wave, pa.one, 1.0f);
             else pass_apply<5, MODE, NW>(pre, pim, pa.b[g], lane, wave, pa.one, 1.0f);
@@ -393,7 +521,13 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
         for (int g = 0; g < pa.nblocks; g++) {
             Ops4<MODE> ops;
             load_ops4<MODE>(ops, pa.b[g].mops, lane);
-            pass_apply4<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, pa.one, 1.0f);
+            if constexpr ((MODE == 2 || MODE == 3) && (TB == 12 || TB == 13)) {
+                if (pa.b[g].pair == PAIR_ROWS) pass_apply4_rows<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, pa.one);
+                else if (pa.b[g].pair == PAIR_COLS) pass_apply4_cols<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, pa.one);
+                else pass_apply4<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, pa.one, 1.0f);
+            } else {
+                pass_apply4<TB, MODE, NW>(pre, pa.b[g], ops, lane, wave, pa.one, 1.0f);
+            }
             __syncthreads();
         }
     }
@@ -500,6 +634,15 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
         pb.mops = mops[g];
         pb.kq = (int)q.size();
         pb.ncolbits = tb - pb.kq;
+        // paired data movement (k = 4 blocks, half-precision products, 12- or
+        // 13-bit tiles): rows when qubit 0 is the block's row bit 0, columns
+        // when the block does not hold qubit 0
+        pb.pair = PAIR_NONE;
+        if (!any5 && (split == 2 || split == 3) && (tb == 12 || tb == 13)) {
+            bool has0 = false;
+            for (int x : q) has0 |= x == 0;
+            pb.pair = !has0 ? PAIR_COLS : (q[0] == 0 ? PAIR_ROWS : PAIR_NONE);
+        }
         for (int r = 0; r < (1 << pb.kq); r++) {
             int e = 0;
             for (int j = 0; j < pb.kq; j++)
@@ -511,6 +654,18 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
             bool inq = false;
             for (int x : q) inq |= (x == lpos[i]);
             if (!inq) pb.colp[c++] = lds_swizzle(1 << i);
+        }
+        // the pairs must be adjacent floats and every 8-byte access aligned
+        if (pb.pair == PAIR_ROWS) {
+            bool ok = pb.erow[1] == 1;
+            for (int r = 0; r < 16; r += 2) ok &= (pb.erow[r] & 1) == 0 && pb.erow[r + 1] == (pb.erow[r] ^ 1);
+            for (int i = 0; i < pb.ncolbits; i++) ok &= (pb.colp[i] & 1) == 0;
+            if (!ok) pb.pair = PAIR_NONE;
+        } else if (pb.pair == PAIR_COLS) {
+            bool ok = pb.colp[0] == 1;
+            for (int r = 0; r < 16; r++) ok &= (pb.erow[r] & 1) == 0;
+            for (int i = 1; i < pb.ncolbits; i++) ok &= (pb.colp[i] & 1) == 0;
+            if (!ok) pb.pair = PAIR_NONE;
         }
     }
     const uint64_t ntiles = s->dim >> tb;

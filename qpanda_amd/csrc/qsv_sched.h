@@ -31,6 +31,7 @@ struct SchedChoice {
 int plan_greedy(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact,
                 std::vector<PlanItem> &plan, bool shape_only = false);
 void widen_to(PlanItem &it, int kq);
+void move_qubit_first(PlanItem &it, int q);
 void widen_with(PlanItem &it, const std::vector<int> &add);
 // Merge narrow blocks into commuting neighbours while the union stays within
 // max_k qubits (the plan's product is unchanged).
