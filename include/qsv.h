@@ -154,6 +154,22 @@ int qsv_device_ptr(qsv_state *s, void **dptr);
 int qsv_stream(qsv_state *s, void **stream);
 int qsv_sync(qsv_state *s);
 
+/* Gate tiling (on != 0): the gate entry points below (and the per-gate path
+ * of qsv_run_circuit) queue their sweeps instead of launching them, as long as
+ * the queued gates' qubits fit one 64 KiB tile together with qubits 0..3 (12
+ * qubits for complex128, 13 for complex64); the next call that is not a gate
+ * (any readout, measurement, copy, qsv_sync, qsv_device_ptr, qsv_set_gate_tiling
+ * with on = 0, ...) applies the queue in one sweep over the state's tiles,
+ * with the same arithmetic, so the state is bit-identical to the
+ * one-sweep-per-gate result.  Gates wider than a tile, dense blocks of 3+
+ * qubits and states smaller than a tile sweep gate by gate.  Off by default;
+ * the first call with on != 0 allocates 96 KiB of device staging.  A caller
+ * that writes the amplitude array through qsv_device_ptr must call qsv_sync
+ * (or any readout) before each such write while tiling is on.  The reference
+ * sweeps the state once per gate (CPUQuantumGates.cpp:92-1435,
+ * GPUGates.cu:129-1020); this call replaces nothing there. */
+int qsv_set_gate_tiling(qsv_state *s, int on);
+
 /* ---- single-qubit and controlled single-qubit gates -------------------------
  * ctrl_mask: bit c set = qubit c is a control that must be 1 (0 = uncontrolled).
  * The target bit must not be in ctrl_mask.  The reference passes controls as a

@@ -84,6 +84,14 @@ QError HipQuantumGates::initState(QuantumGateParam *p)
         mState = nullptr;
         return undefineError;
     }
+    // Runs of gates that share a tile go through HBM once; the next readout
+    // (measure, pMeasure, getQState, ...) applies what is queued, and the
+    // state is bit-identical to one sweep per gate.
+    if (qsv_set_gate_tiling(mState, 1) != QSV_OK) {
+        qsv_destroy(mState);
+        mState = nullptr;
+        return undefineError;
+    }
     mQubits = n;
     return qErrorNone;
 }

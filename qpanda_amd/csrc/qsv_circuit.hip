@@ -241,6 +241,10 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
     if ((rc = max_mat_index(gates, ngates, &need))) return rc;
     if (need && !mats) return fail(QSV_ERR_ARG, "matrix pool missing");
     if ((rc = upload_mats(s, mats, need))) return rc;
+    if ((rc = tiles_flush(s))) return rc;
+    // The fused path launches its own sweeps; a per-record profile times each
+    // record's own sweep.  Otherwise gate tiling (when on) queues the gates.
+    TileSuspend suspend(s, o.fuse_k >= 2 || o.profile != 0);
 
     hipEvent_t e0, e1;
     QSV_HIP(hipEventCreate(&e0));
@@ -265,6 +269,7 @@ extern "C" int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngat
             }
         }
     }
+    if (!rc) rc = tiles_flush(s);
     hipError_t er = hipEventRecord(e1, s->stream);
     if (!rc && er != hipSuccess) rc = hip_fail(er, "hipEventRecord");
     if (!rc) {

@@ -4,6 +4,7 @@
 // Every entry point validates its qubits on the host before any launch, so no
 // kernel ever receives an index outside the state.
 #include "qsv_internal.h"
+#include "qsv_gatemath.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -66,8 +67,8 @@ static int check_mask(const qsv_state *s, uint64_t ctrl, uint64_t targets)
 
 using namespace qsv;
 
-#define CHECK_STATE(s) \
-    do { if (!(s) || !(s)->amps) return fail(QSV_ERR_ARG, "null state"); } while (0)
+#define CHECK_STATE(s) QSV_CHECK_STATE(s)
+#define CHECK_GATE_STATE(s) QSV_CHECK_GATE_STATE(s)
 
 extern "C" {
 
@@ -126,6 +127,7 @@ int qsv_create(int n, int precision, qsv_state **out)
 int qsv_destroy(qsv_state *s)
 {
     if (!s) return QSV_OK;
+    tiles_free(s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->amps) (void)hipFree(s->amps);
     if (s->red) (void)hipFree(s->red);
@@ -138,7 +140,8 @@ int qsv_destroy(qsv_state *s)
 
 int qsv_reset_zero(qsv_state *s)
 {
-    CHECK_STATE(s);
+    CHECK_GATE_STATE(s);
+    tiles_discard(s);                         // queued gates would act on the old state
     return launch_fill_zero(s);
 }
 
@@ -204,7 +207,7 @@ static const double kSq2Exact = 0.70710678118654752440;
 
 static int one_qubit(qsv_state *s, int op, int t, uint64_t ctrl, const double *u, double h)
 {
-    CHECK_STATE(s);
+    CHECK_GATE_STATE(s);
     int rc = check_qubit(s, t);
     if (rc) return rc;
     const uint64_t tb = uint64_t(1) << t;
@@ -264,7 +267,7 @@ int qsv_rz(qsv_state *s, int target, double theta, int dagger, uint64_t ctrl_mas
 
 int qsv_cphase(qsv_state *s, int q1, int q2, double re, double im, uint64_t ctrl_mask)
 {
-    CHECK_STATE(s);
+    CHECK_GATE_STATE(s);
     int rc;
     if ((rc = check_qubit(s, q1)) || (rc = check_qubit(s, q2))) return rc;
     if (q1 == q2) return QSV_OK;            // the reference's loop is empty (CPUQuantumGates.cpp:813)
@@ -282,7 +285,7 @@ int qsv_cr(qsv_state *s, int q1, int q2, double theta, int dagger)
 
 static int iswap_like(qsv_state *s, int q1, int q2, const double *u)
 {
-    CHECK_STATE(s);
+    CHECK_GATE_STATE(s);
     int rc;
     if ((rc = check_qubit(s, q1)) || (rc = check_qubit(s, q2))) return rc;
     if (q1 == q2) return QSV_OK;            // empty loop in the reference
@@ -308,7 +311,7 @@ int qsv_sqiswap(qsv_state *s, int q1, int q2, int dagger, int exact)
 
 int qsv_cswap(qsv_state *s, int c, int a, int b)
 {
-    CHECK_STATE(s);
+    CHECK_GATE_STATE(s);
     int rc;
     if ((rc = check_qubit(s, c)) || (rc = check_qubit(s, a)) || (rc = check_qubit(s, b))) return rc;
     if (c == a || c == b || a == b) return fail(QSV_ERR_QUBIT, "controlSwap qubits must differ");
@@ -319,7 +322,7 @@ int qsv_cswap(qsv_state *s, int c, int a, int b)
 
 int qsv_apply_2q(qsv_state *s, int qa, int qb, const double *u)
 {
-    CHECK_STATE(s);
+    CHECK_GATE_STATE(s);
     if (!u) return fail(QSV_ERR_ARG, "null matrix");
     int rc;
     if ((rc = check_qubit(s, qa)) || (rc = check_qubit(s, qb))) return rc;

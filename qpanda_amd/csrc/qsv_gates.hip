@@ -16,7 +16,7 @@
 // bytes whenever the lowest inserted position is >= 6) and several loads are
 // in flight per thread.  The sweeps are HBM-bound: 2 x 16 B read + written per
 // pair for c128.
-#include "qsv_internal.h"
+#include "qsv_gatemath.h"
 
 namespace qsv {
 
@@ -24,28 +24,11 @@ namespace {
 
 constexpr int kPairsPerThread = 4;
 
-template <typename T> struct Real;
-template <> struct Real<c128> { using type = double; };
-template <> struct Real<c64> { using type = float; };
-
-template <typename R> struct Coef {
-    R u[8];
-};
-
-template <typename T>
-__device__ __forceinline__ T cmadd(T acc, typename Real<T>::type ur, typename Real<T>::type ui, T x)
-{
-    acc.x += ur * x.x - ui * x.y;
-    acc.y += ur * x.y + ui * x.x;
-    return acc;
-}
-
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_pair(T *__restrict__ a, uint64_t npairs, InsertSet ins,
                                                   uint64_t or0, uint64_t or1,
                                                   Coef<typename Real<T>::type> c)
 {
-    using R = typename Real<T>::type;
     const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kPairsPerThread) + threadIdx.x;
     T x0[kPairsPerThread], x1[kPairsPerThread];
     uint64_t i0[kPairsPerThread];
@@ -64,20 +47,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(T *__restrict__ a, uint64_t npa
     for (int k = 0; k < kPairsPerThread; k++) {
         if (!ok[k]) continue;
         T y0, y1;
-        if (OP == OP_H) {
-            const R h = c.u[0];
-            y0.x = (x0[k].x + x1[k].x) * h; y0.y = (x0[k].y + x1[k].y) * h;
-            y1.x = (x0[k].x - x1[k].x) * h; y1.y = (x0[k].y - x1[k].y) * h;
-        } else if (OP == OP_X) {
-            y0 = x1[k]; y1 = x0[k];
-        } else if (OP == OP_DIAG) {
-            y0.x = c.u[0] * x0[k].x - c.u[1] * x0[k].y; y0.y = c.u[0] * x0[k].y + c.u[1] * x0[k].x;
-            y1.x = c.u[2] * x1[k].x - c.u[3] * x1[k].y; y1.y = c.u[2] * x1[k].y + c.u[3] * x1[k].x;
-        } else {
-            y0.x = 0; y0.y = 0; y1.x = 0; y1.y = 0;
-            y0 = cmadd(y0, c.u[0], c.u[1], x0[k]); y0 = cmadd(y0, c.u[2], c.u[3], x1[k]);
-            y1 = cmadd(y1, c.u[4], c.u[5], x0[k]); y1 = cmadd(y1, c.u[6], c.u[7], x1[k]);
-        }
+        pair_math<T, OP>(x0[k], x1[k], c, y0, y1);
         a[i0[k] | or0] = y0;
         a[i0[k] | or1] = y1;
     }
@@ -96,10 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(T *__restrict__ a, uint64_t co
         const uint64_t p = base + (uint64_t)k * kBlock;
         if (p >= count) continue;
         const uint64_t i = expand_index(p, ins) | orm;
-        T x = a[i], y;
-        y.x = re * x.x - im * x.y;
-        y.y = re * x.y + im * x.x;
-        a[i] = y;
+        a[i] = phase_math(a[i], re, im);
     }
 }
 
@@ -131,21 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_block_f64(T *__restrict__ a, uint64_
                 xr[c] = v.x; xi[c] = v.y;
             }
         }
-        for (int r = 0; r < d; r++) {
-            double sr = 0, si = 0;
-#pragma unroll
-            for (int c = 0; c < 32; c++) {
-                if (c < d) {
-                    const double ur = su[2 * (r * d + c)], ui = su[2 * (r * d + c) + 1];
-                    sr += ur * xr[c] - ui * xi[c];
-                    si += ur * xi[c] + ui * xr[c];
-                }
-            }
-            T y;
-            y.x = (typename Real<T>::type)sr;
-            y.y = (typename Real<T>::type)si;
-            a[b + ba.off[r]] = y;
-        }
+        for (int r = 0; r < d; r++) a[b + ba.off[r]] = block_row_math<T, 32>(su, d, r, xr, xi);
     }
 }
 
@@ -176,6 +129,10 @@ inline unsigned grid_for(uint64_t items, uint64_t per_block)
 int launch_pair(qsv_state *s, int op, const InsertSet &ins, uint64_t or0, uint64_t or1,
                 const double *u, double hscale)
 {
+    if (s->tq) {
+        const int q = tiles_enqueue_pair(s, op, ins, or0, or1, u, hscale);
+        if (q <= 0) return q;                  // queued, or an error
+    }
     const uint64_t npairs = s->dim >> ins.n;
     const unsigned grid = grid_for(npairs, (uint64_t)kBlock * kPairsPerThread);
     if (s->prec == QSV_C128) {
@@ -210,6 +167,10 @@ int launch_pair(qsv_state *s, int op, const InsertSet &ins, uint64_t or0, uint64
 
 int launch_phase(qsv_state *s, const InsertSet &ins, uint64_t orm, double re, double im)
 {
+    if (s->tq) {
+        const int q = tiles_enqueue_phase(s, ins, orm, re, im);
+        if (q <= 0) return q;
+    }
     const uint64_t count = s->dim >> ins.n;
     const unsigned grid = grid_for(count, (uint64_t)kBlock * kPairsPerThread);
     if (s->prec == QSV_C128)
@@ -233,6 +194,10 @@ int launch_block(qsv_state *s, const int *qubits, int k, const double *u, uint64
 int launch_block_dev(qsv_state *s, const int *qubits, int k, const double *u_dev, uint64_t ctrl_mask)
 {
     if (k < 1 || k > 5) return fail(QSV_ERR_ARG, "block size must be 1..5");
+    if (s->tq) {
+        const int q = tiles_enqueue_block(s, qubits, k, u_dev, ctrl_mask);
+        if (q <= 0) return q;
+    }
     BlockArgs ba{};
     ba.d = 1 << k;
     ba.ctrl = ctrl_mask;

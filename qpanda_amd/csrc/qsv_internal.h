@@ -74,6 +74,7 @@ struct qsv_state {
     int64_t tie_resolutions = 0;
     float f16_scale = 32768.0f;      // FP16x3 products: state scale (f16_state_scale)
     struct qsv_costmodel *cm = nullptr;   // attached cost model (not owned)
+    struct TileQueue *tq = nullptr;       // gate tiling (qsv_set_gate_tiling), owned
     std::vector<float> tlog_feats;        // per-block features of the last profiled fused run
     std::vector<float> tlog_ms;           // and their measured device ms
 };
@@ -86,14 +87,45 @@ size_t amp_bytes(const qsv_state *s);
 int check_qubit(const qsv_state *s, int q);
 int ensure_aux(qsv_state *s, size_t bytes);
 
-// Gate launchers (qsv_gates.hip).  ops for the pair kernel:
-enum PairOp { OP_H = 0, OP_X = 1, OP_U = 2, OP_DIAG = 3 };
+// Gate launchers (qsv_gates.hip); pair kernel operations in qsv_gatemath.h.
 int launch_pair(qsv_state *s, int op, const InsertSet &ins, uint64_t or0, uint64_t or1,
                 const double *u, double hscale);
 int launch_phase(qsv_state *s, const InsertSet &ins, uint64_t orm, double re, double im);
 int launch_block(qsv_state *s, const int *qubits, int k, const double *u, uint64_t ctrl_mask);
 int launch_block_dev(qsv_state *s, const int *qubits, int k, const double *u_dev, uint64_t ctrl_mask);
 int launch_fill_zero(qsv_state *s);
+
+// Gate tiling (qsv_tiles.hip): while a state's queue is active, the pair /
+// phase / small-block launchers hand their launch to tiles_enqueue_*, which
+// returns 0 when it queued it and 1 when the caller must launch it itself.
+// Every entry point that is not a gate flushes the queue first.
+int tiles_enqueue_pair(qsv_state *s, int op, const InsertSet &ins, uint64_t or0, uint64_t or1, const double *u,
+                       double hscale);
+int tiles_enqueue_phase(qsv_state *s, const InsertSet &ins, uint64_t orm, double re, double im);
+int tiles_enqueue_block(qsv_state *s, const int *qubits, int k, const double *u_dev, uint64_t ctrl_mask);
+int tiles_flush(qsv_state *s);
+void tiles_discard(qsv_state *s);
+void tiles_free(qsv_state *s);
+// Entry-point state check: a null state is an argument error, and gates that
+// tiling has queued are applied before anything else reads or writes the
+// state.  The gate entry points check with QSV_CHECK_GATE_STATE and queue.
+#define QSV_CHECK_STATE(s)                                                   \
+    do {                                                                     \
+        if (!(s) || !(s)->amps) return fail(QSV_ERR_ARG, "null state");    \
+        if ((s)->tq) {                                                       \
+            const int rc_flush_ = tiles_flush(s);                            \
+            if (rc_flush_) return rc_flush_;                                 \
+        }                                                                    \
+    } while (0)
+#define QSV_CHECK_GATE_STATE(s) \
+    do { if (!(s) || !(s)->amps) return fail(QSV_ERR_ARG, "null state"); } while (0)
+// Suspends tiling while it lives when `on` (the fused path launches directly).
+struct TileSuspend {
+    TileSuspend(qsv_state *s, bool on);
+    ~TileSuspend();
+    qsv_state *s_;
+    bool was_;
+};
 
 // Reductions (qsv_readout.hip).
 int reduce_prob0(qsv_state *s, int q, double *p0);

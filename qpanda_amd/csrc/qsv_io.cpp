@@ -189,8 +189,7 @@ const char *parse_line(const char *p, uint64_t want, double *re, double *im)
 
 using namespace qsv;
 
-#define CHECK_STATE(s) \
-    do { if (!(s) || !(s)->amps) return fail(QSV_ERR_ARG, "null state"); } while (0)
+#define CHECK_STATE(s) QSV_CHECK_STATE(s)
 
 int qsv_set_state(qsv_state *s, const double *amps)
 {

@@ -295,6 +295,7 @@ static int make_work(qsv_state *s, int prec, qsv_state *w)
     *w = *s;
     w->prec = prec;
     w->amps = nullptr;
+    w->tq = nullptr;                     // the work copy applies nothing deferred
     const size_t bytes = s->dim * (prec == QSV_C128 ? sizeof(c128) : sizeof(c64));
     if (hipMalloc(&w->amps, bytes) != hipSuccess) {
         w->amps = nullptr;

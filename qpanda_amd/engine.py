@@ -22,6 +22,7 @@ FILE_BINARY, FILE_TEXT = 0, 1
 EXPORTED = [
     "qsv_create", "qsv_destroy", "qsv_reset_zero", "qsv_num_qubits", "qsv_precision",
     "qsv_set_state", "qsv_get_state", "qsv_save_state", "qsv_load_state", "qsv_device_ptr", "qsv_stream", "qsv_sync",
+    "qsv_set_gate_tiling",
     "qsv_h", "qsv_x", "qsv_apply_1q", "qsv_diag", "qsv_rx", "qsv_ry", "qsv_rz",
     "qsv_cphase", "qsv_cr", "qsv_iswap", "qsv_sqiswap", "qsv_cswap", "qsv_apply_2q",
     "qsv_apply_block", "qsv_prob0", "qsv_norm2", "qsv_collapse", "qsv_measure",
@@ -94,6 +95,7 @@ def lib():
             "qsv_device_ptr": ([vp, C.POINTER(vp)], I),
             "qsv_stream": ([vp, C.POINTER(vp)], I),
             "qsv_sync": ([vp], I),
+            "qsv_set_gate_tiling": ([vp, I], I),
             "qsv_h": ([vp, I, U64, I], I),
             "qsv_x": ([vp, I, U64], I),
             "qsv_apply_1q": ([vp, I, P, U64], I),
@@ -231,6 +233,11 @@ class StateVector:
 
     def sync(self):
         _check(lib().qsv_sync(self.h))
+
+    def set_gate_tiling(self, on: bool = True):
+        """Queue gate sweeps and apply runs of gates whose qubits share a
+        64 KiB tile in one sweep (bit-identical state); any readout flushes."""
+        _check(lib().qsv_set_gate_tiling(self.h, 1 if on else 0))
 
     def attach_costmodel(self, model: "CostModel | None"):
         self._cm = model                      # keep the model alive while attached
