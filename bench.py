@@ -231,6 +231,22 @@ def cpu_baseline(circ, seconds, stride):
                 "sample": f"oracle restatement, 1 thread, n={circ.n}, {done} strided gates in {dt:.1f} s"}
 
 
+def tiled_plain_run(E, circ, sv_plain, plain):
+    """The plain path with gate tiling on (queued per-gate sweeps applied over
+    64 KiB LDS tiles): one run, its device time, and a strided sample of its
+    amplitudes compared bit for bit with the plain path's."""
+    log("plain path with gate tiling: one run")
+    with E.StateVector(circ.n, E.C128) as sv:
+        sv.set_gate_tiling(True)
+        st = sv.run(circ)
+        idx = np.arange(0, 1 << circ.n, max(1, (1 << circ.n) // 65536), dtype=np.uint64)
+        same = bool(np.array_equal(sv.gather(idx), sv_plain.gather(idx)))
+    G = circ.ngates
+    return {"gate_apps_per_s": G / (st.device_ms / 1e3), "device_ms": st.device_ms, "sweeps": int(st.launches),
+            "speedup_vs_plain": plain.device_ms / st.device_ms, "sample_equals_plain": same,
+            "sample": f"{idx.size} amplitudes at stride {int(idx[1] - idx[0]) if idx.size > 1 else 1}"}
+
+
 def copy_rate(n, reps=5):
     """State-to-state copy of a complex64 state of n qubits (read + write), GB/s."""
     from qpanda_amd import engine as E
@@ -360,6 +376,7 @@ def main():
         if sv_plain is not None:
             plain = run_plain(True)
             sv_plain.sync()
+            result["plain_tiled"] = tiled_plain_run(E, circ, sv_plain, plain)
         sv = E.StateVector(a.n, E.C64)
         if cm is not None:
             sv.attach_costmodel(cm)
