@@ -18,8 +18,9 @@ Paths:
          of blocks into shared HBM sweeps, tile shape); --scheduler greedy
          gives one sweep per block, --scheduler window one sweep per window.
 A step is one full circuit execution from |0...0> on the headline path (fused).
-The plain path and the greedy k=4 fused path each run once for comparison;
-fidelity |<plain|fused>|^2 is computed on the device in FP64.
+The plain path (also once with gate tiling) and the greedy k=4 fused path with
+the FP32 split product each run once for comparison; fidelity |<plain|fused>|^2
+is computed on the device in FP64.
 
 Other workloads (one JSON line each; not the default line):
   --workload large33  33-qubit random circuit of depth 60, complex64 (64 GiB),
@@ -37,6 +38,10 @@ Other workloads (one JSON line each; not the default line):
                       The reference re-runs the program per shot; the facade
                       runs it once and samples the shots, and the bench checks
                       on a short run that both give the same result strings;
+  --workload lowq     gates on the lowest qubits of a 30-qubit complex128 state
+                      (CZ(0,1), CNOT(0,1), H(0); CZ(20,21) for comparison), 32
+                      of each: ms per gate one sweep per gate, and with gate
+                      tiling (the 32 gates in one sweep over LDS tiles);
   --workload stateio  whole-state host copies and binary state files of a
                       28-qubit complex128 state (4 GiB) and its complex64 form:
                       GB/s of get / set / save / load, checked bit-exact.
@@ -82,7 +87,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", choices=["random30", "large33", "batch", "qrunes", "stateio"], default="random30")
+    p.add_argument("--workload", choices=["random30", "large33", "batch", "qrunes", "stateio", "lowq"],
+                   default="random30")
     p.add_argument("--n", type=int, default=None, help="qubits (random30: 30, large33: 33)")
     p.add_argument("--depth", type=int, default=None, help="layers (random30: 200, large33: 60)")
     p.add_argument("--shots", type=int, default=100000, help="large33, qrunes: measurement shots")
@@ -337,6 +343,8 @@ def main():
         return main_qrunes(a)
     if a.workload == "stateio":
         return main_stateio(a)
+    if a.workload == "lowq":
+        return main_lowq(a)
     from qpanda_amd import circuits as C
     from qpanda_amd import engine as E
 
@@ -382,10 +390,13 @@ def main():
             sv.attach_costmodel(cm)
         k = a.fuse_k
         if sv_plain is not None and sched != 0:
-            log("greedy k=4 fused path: one run (comparison)")
-            greedy = sv.run(circ, fuse_k=4, split=a.split, scheduler=0, profile=True)
+            # the FP32 split-precision product (FP32 MFMA + BF16 correction of
+            # the matrix's lo part), one greedy k = 4 block per sweep
+            log("greedy k=4 fused path, FP32 split product: one run (comparison)")
+            greedy = sv.run(circ, fuse_k=4, split=1, scheduler=0, profile=True)
             result["greedy_k4"] = {"gate_apps_per_s": G / (greedy.device_ms / 1e3), "device_ms": greedy.device_ms,
-                                   "sweeps": int(greedy.launches), "fidelity_vs_fp64": sv_plain.fidelity(sv)}
+                                   "sweeps": int(greedy.launches), "product": "split",
+                                   "fidelity_vs_fp64": sv_plain.fidelity(sv)}
         for _ in range(warmup):
             sv.reset()
             sv.run(circ, fuse_k=k, split=a.split, scheduler=sched, group=a.group, pass_tb=a.pass_tb)
@@ -654,6 +665,43 @@ def main_qrunes(a):
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+def main_lowq(a):
+    """Per-gate sweeps of low-qubit gates against the same gates tiled."""
+    from qpanda_amd import circuits as C
+    from qpanda_amd import engine as E
+    n, reps = a.n, 32
+    N = float(1 << n)
+    cases = {"CZ(0,1)": (C.CZ, 0, 1, N / 4), "CNOT(0,1)": (C.CNOT, 0, 1, N / 2), "H(0)": (C.H, 0, 0, N),
+             "CZ(20,21)": (C.CZ, 20, 21, N / 4)}
+    out = {}
+    with E.StateVector(n, E.C128) as sv:
+        for name, (kind, q0, q1, touched) in cases.items():
+            b = C._Builder(n)
+            for _ in range(reps):
+                b.add(kind, q0, q1)
+            circ = b.build()
+            sv.set_gate_tiling(False)
+            sv.run(circ)                                   # warm-up
+            per = sv.run(circ, profile=True).device_ms / reps
+            sv.set_gate_tiling(True)
+            tiled = sv.run(circ).device_ms / reps
+            sv.set_gate_tiling(False)
+            alg = 2 * 16 * touched
+            out[name] = {"per_gate_ms": per, "per_gate_alg_tbs": alg / (per * 1e9),
+                         "tiled_ms_per_gate": tiled, "speedup": per / tiled}
+            log(f"{name}: per-gate {per:.3f} ms ({alg / (per * 1e9):.2f} TB/s algorithmic), "
+                f"tiled {tiled:.3f} ms per gate")
+    line = {
+        "metric": f"ms per low-qubit gate at {n}q (c128), per-gate sweep vs gate tiling",
+        "value": out["CZ(0,1)"]["tiled_ms_per_gate"], "unit": "ms per gate", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": out["CZ(0,1)"]["tiled_ms_per_gate"] * reps, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic gate runs",
+        "config": {"workload": f"{reps} x each gate, n={n}", "n_qubits": n},
+        "roofline": None, "cpu_baseline": None, "cases": out,
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -20,7 +20,16 @@
 //            |0...0> without reading it and the last pass of the last layer
 //            reduces instead of storing: (4 L - 2) x 8 x 2^n bytes of HBM
 //            traffic per instance.
-// The counter is reset by a one-thread kernel (no memset node), so the pair of
+// Instances of 20..22 qubits are split over 2, 4 or 8 workgroups ("parts",
+// consecutive work items): every part takes an equal share of the tiles of
+// each pass, the parts meet at a per-instance barrier (an arrival counter in
+// HBM) after every pass, and each adds its share of the ZZ sum to a per-part
+// slot that part 0 sums in part order.  The state lives in part 0's slice.
+// Parts are handed out in item order, so at most one instance is ever partly
+// handed out: every other part that waits at a barrier waits for parts that
+// are already running, and the waits end (bounded spins that raise an error
+// flag as a safety net).
+// The counters are reset by a small kernel (no memset node), so the pair of
 // launches can be captured in a hipGraph and replayed.
 #include "qsv_internal.h"
 
@@ -37,10 +46,30 @@ namespace {
 constexpr int kLdsBits = 14;                 // 2^14 complex64 = 128 KiB
 constexpr int kMaxBatchQubits = 22;
 
+constexpr int kSplitMinQubits = 20;          // instances from here on run as several parts
+constexpr int kMaxParts = 8;
+constexpr unsigned kSpinLimit = 1u << 26;    // barrier polls before the error flag (seconds)
+
 struct Inst {
     int n, layers;
     int64_t mat_off;                         // float offset of this instance's U's (layers x n x 8)
     int64_t index;                           // caller's instance index
+    int parts;                               // workgroups sharing the instance
+};
+
+struct Item {                                // one work item: a part of an instance
+    int inst;                                // index into the (sorted) instance array
+    int part;
+};
+
+struct BatchSync {
+    const Item *items;
+    int64_t nitems;
+    unsigned int *counter;                   // next work item
+    unsigned int *arrive;                    // per instance: barrier arrivals
+    unsigned int *owner;                     // per instance: 1 + the workgroup whose slice holds it
+    double *partial;                         // per instance: kMaxParts partial ZZ sums
+    unsigned int *err;                       // set when a bounded wait gave up
 };
 
 
@@ -183,31 +212,75 @@ __device__ __forceinline__ double zz_weight(uint64_t x, int n)
     return (double)(n - 1 - 2 * flips);
 }
 
-__global__ void k_batch_reset(unsigned int *counter) { *counter = 0; }
+__global__ void k_batch_reset(BatchSync bs, int64_t count)
+{
+    for (int64_t i = threadIdx.x; i < count; i += blockDim.x) {
+        bs.arrive[i] = 0;
+        bs.owner[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+        *bs.counter = 0;
+        *bs.err = 0;
+    }
+}
+
+// Thread 0 polls *word until it reaches target (acquire, device scope); the
+// workgroup waits for it.  Gives up after kSpinLimit polls and raises err.
+__device__ __forceinline__ unsigned int wait_at_least(unsigned int *word, unsigned int target, unsigned int *err)
+{
+    unsigned int v = __hip_atomic_load(word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned int spins = 0; v < target; spins++) {
+        if (spins >= kSpinLimit) {
+            atomicOr(err, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+        v = __hip_atomic_load(word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return v;
+}
+
+// Barrier of the parts of one instance: this workgroup's global writes
+// complete and are released at device scope before it arrives; it proceeds
+// once all parts have arrived `round` times.
+__device__ __forceinline__ void parts_barrier(unsigned int *arrive, unsigned int parts, unsigned int round,
+                                              unsigned int *err)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(arrive, 1u);
+        wait_at_least(arrive, parts * round, err);
+        __threadfence();
+    }
+    __syncthreads();
+}
 
 template <int T>
-__global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, int64_t count,
+__global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, BatchSync bs,
                                                  const float *__restrict__ mats, float2 *__restrict__ scratch,
-                                                 unsigned int *__restrict__ counter, double *__restrict__ zz,
-                                                 uint64_t *__restrict__ t_start, uint64_t *__restrict__ t_end)
+                                                 double *__restrict__ zz, uint64_t *__restrict__ t_start,
+                                                 uint64_t *__restrict__ t_end)
 {
     // exactly 128 KiB of LDS: the work-counter slot and the reduction scratch
     // reuse the tile while it holds no state (between instances)
     __shared__ __align__(16) float2 tile[1 << kLdsBits];
     unsigned int *const next_slot = reinterpret_cast<unsigned int *>(tile);
     double *const red = reinterpret_cast<double *>(tile);
-    float2 *slice = scratch + (size_t)blockIdx.x * ((size_t)1 << kMaxBatchQubits);
     for (;;) {
-        if (threadIdx.x == 0) *next_slot = atomicAdd(counter, 1u);
+        if (threadIdx.x == 0) *next_slot = atomicAdd(bs.counter, 1u);
         __syncthreads();
         const unsigned int k = *next_slot;
         __syncthreads();
-        if (k >= count) break;                // every wave of the workgroup sees the same k
-        const Inst I = inst[k];
+        if (k >= bs.nitems) break;            // every wave of the workgroup sees the same k
+        const Item it = bs.items[k];
+        const Inst I = inst[it.inst];
         const int n = I.n;
+        const unsigned int parts = (unsigned int)I.parts;
         const float *U = mats + I.mat_off;
-        if (threadIdx.x == 0) t_start[I.index] = wall_clock64();
+        if (threadIdx.x == 0 && it.part == 0) t_start[I.index] = wall_clock64();
         double acc = 0;
+        unsigned int round = 0;               // barriers passed (split instances)
         if (n <= kLdsBits) {
             const uint32_t dim = 1u << n;
             for (uint32_t i = threadIdx.x; i < dim; i += T) tile[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
@@ -226,12 +299,31 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
             constexpr uint32_t TS = 1u << kLdsBits;
             constexpr int NL = (int)TS / (2 * T);          // 16-byte accesses per thread per tile
             const int c = 27 - n;                           // low bits in the high-qubit tile
+            // the slice: this workgroup's own, or part 0's for a split instance
+            unsigned int owner = blockIdx.x;
+            if (parts > 1) {
+                if (threadIdx.x == 0) {
+                    if (it.part == 0)
+                        __hip_atomic_store(&bs.owner[it.inst], blockIdx.x + 1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        *next_slot = wait_at_least(&bs.owner[it.inst], 1u, bs.err) - 1;
+                }
+                __syncthreads();
+                if (it.part != 0) owner = *next_slot;
+                __syncthreads();
+                if (owner >= gridDim.x) owner = blockIdx.x;     // only after a failed wait
+            }
+            float2 *slice = scratch + (size_t)owner * ((size_t)1 << kMaxBatchQubits);
+            // this part's share of each pass's tiles
+            const uint64_t share = (dim >> kLdsBits) / parts;
+            const uint64_t t_lo = share * (uint64_t)it.part, t_hi = t_lo + share;
             if (I.layers == 0 && threadIdx.x == 0) acc = zz_weight(0, n);
             for (int l = 0; l < I.layers; l++) {
                 const float *Ul = U + (size_t)l * n * 8;
                 const bool last = l + 1 == I.layers;
                 // pass A: qubits 0..13, contiguous tiles
-                for (uint64_t base = 0; base < dim; base += TS) {
+                for (uint64_t base = t_lo * TS; base < t_hi * TS; base += TS) {
                     if (l == 0) {
                         for (uint32_t e = threadIdx.x; e < TS; e += T)
                             tile[e] = make_float2(base == 0 && e == 0 ? 1.f : 0.f, 0.f);
@@ -254,10 +346,10 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
                         *reinterpret_cast<float4 *>(slice + base + 2 * (threadIdx.x + T * i)) = out[i];
                     __syncthreads();
                 }
+                if (parts > 1) parts_barrier(&bs.arrive[it.inst], parts, ++round, bs.err);
                 // pass B: tile bits 0..c-1 = qubits 0..c-1, bits c.. = qubits 13..n-1;
                 // c >= 5, so tile elements e, e + 1 (e even) are adjacent in the slice
-                const uint64_t ntile = dim >> kLdsBits;
-                for (uint64_t t = 0; t < ntile; t++) {
+                for (uint64_t t = t_lo; t < t_hi; t++) {
                     // t enumerates qubits c..12 (13 - c = n - 14 bits)
                     const uint64_t fixed = t << c;
                     auto gaddr = [&](uint32_t e) -> uint64_t {
@@ -292,13 +384,27 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
                     }
                     __syncthreads();
                 }
+                if (parts > 1 && !last) parts_barrier(&bs.arrive[it.inst], parts, ++round, bs.err);
             }
         }
         __syncthreads();                      // the tile is read; reuse it as scratch
         const double s = block_sum_d(acc, red);
-        if (threadIdx.x == 0) {
-            zz[I.index] = s;
-            t_end[I.index] = wall_clock64();
+        if (parts == 1) {
+            if (threadIdx.x == 0) {
+                zz[I.index] = s;
+                t_end[I.index] = wall_clock64();
+            }
+        } else {
+            // every part's share, then part 0 sums them in part order (the
+            // last barrier also keeps part 0's slice until all parts are done)
+            if (threadIdx.x == 0) bs.partial[(size_t)it.inst * kMaxParts + it.part] = s;
+            parts_barrier(&bs.arrive[it.inst], parts, ++round, bs.err);
+            if (threadIdx.x == 0 && it.part == 0) {
+                double z = 0;
+                for (unsigned int j = 0; j < parts; j++) z += bs.partial[(size_t)it.inst * kMaxParts + j];
+                zz[I.index] = z;
+                t_end[I.index] = wall_clock64();
+            }
         }
         __syncthreads();
     }
@@ -310,9 +416,12 @@ __global__ __launch_bounds__(T) void k_batch_hea(const Inst *__restrict__ inst, 
 struct qsv_batch {
     int64_t count = 0;
     qsv::Inst *d_inst = nullptr;
+    qsv::Item *d_items = nullptr;
+    int64_t nitems = 0;
     float *d_mats = nullptr;
     float2 *d_scratch = nullptr;
-    unsigned int *d_counter = nullptr;
+    unsigned int *d_counter = nullptr;       // [0] next item, [1] error flag, then arrivals and owners
+    double *d_partial = nullptr;
     double *d_zz = nullptr;
     uint64_t *d_t0 = nullptr, *d_t1 = nullptr;
     int grid = 0;
@@ -330,21 +439,35 @@ static void batch_free(qsv_batch *b)
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    for (void *p : {(void *)b->d_inst, (void *)b->d_mats, (void *)b->d_scratch, (void *)b->d_counter,
-                    (void *)b->d_zz, (void *)b->d_t0, (void *)b->d_t1})
+    for (void *p : {(void *)b->d_inst, (void *)b->d_items, (void *)b->d_mats, (void *)b->d_scratch,
+                    (void *)b->d_counter, (void *)b->d_partial, (void *)b->d_zz, (void *)b->d_t0, (void *)b->d_t1})
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
+static BatchSync batch_sync(qsv_batch *b)
+{
+    BatchSync bs;
+    bs.items = b->d_items;
+    bs.nitems = b->nitems;
+    bs.counter = b->d_counter;
+    bs.err = b->d_counter + 1;
+    bs.arrive = b->d_counter + 2;
+    bs.owner = b->d_counter + 2 + b->count;
+    bs.partial = b->d_partial;
+    return bs;
+}
+
 static int batch_launch(qsv_batch *b, int threads)
 {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, b->stream, b->d_counter);
+    const BatchSync bs = batch_sync(b);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, b->stream, bs, b->count);
     const dim3 g(b->grid);
     switch (threads) {
-    case 256: hipLaunchKernelGGL(k_batch_hea<256>, g, dim3(256), 0, b->stream, b->d_inst, b->count, b->d_mats, b->d_scratch, b->d_counter, b->d_zz, b->d_t0, b->d_t1); break;
-    case 512: hipLaunchKernelGGL(k_batch_hea<512>, g, dim3(512), 0, b->stream, b->d_inst, b->count, b->d_mats, b->d_scratch, b->d_counter, b->d_zz, b->d_t0, b->d_t1); break;
-    default: hipLaunchKernelGGL(k_batch_hea<1024>, g, dim3(1024), 0, b->stream, b->d_inst, b->count, b->d_mats, b->d_scratch, b->d_counter, b->d_zz, b->d_t0, b->d_t1); break;
+    case 256: hipLaunchKernelGGL(k_batch_hea<256>, g, dim3(256), 0, b->stream, b->d_inst, bs, b->d_mats, b->d_scratch, b->d_zz, b->d_t0, b->d_t1); break;
+    case 512: hipLaunchKernelGGL(k_batch_hea<512>, g, dim3(512), 0, b->stream, b->d_inst, bs, b->d_mats, b->d_scratch, b->d_zz, b->d_t0, b->d_t1); break;
+    default: hipLaunchKernelGGL(k_batch_hea<1024>, g, dim3(1024), 0, b->stream, b->d_inst, bs, b->d_mats, b->d_scratch, b->d_zz, b->d_t0, b->d_t1); break;
     }
     QSV_HIP(hipGetLastError());
     return QSV_OK;
@@ -367,8 +490,20 @@ static int batch_timed(qsv_batch *b, int threads, double *ms)
     return rc;
 }
 
+// A bounded wait that gave up leaves the results undefined: report it.
+static int batch_check(qsv_batch *b)
+{
+    unsigned int err = 0;
+    QSV_HIP(hipMemcpyAsync(&err, b->d_counter + 1, sizeof(err), hipMemcpyDeviceToHost, b->stream));
+    QSV_HIP(hipStreamSynchronize(b->stream));
+    if (err) return fail(QSV_ERR_DEVICE, "batch: a part of a split instance waited too long at its barrier");
+    return QSV_OK;
+}
+
 static int batch_results(qsv_batch *b, double *zz, double *lat_us)
 {
+    int rc = batch_check(b);
+    if (rc) return rc;
     if (zz) QSV_HIP(hipMemcpyAsync(zz, b->d_zz, sizeof(double) * b->count, hipMemcpyDeviceToHost, b->stream));
     std::vector<uint64_t> t0, t1;
     if (lat_us) {
@@ -401,7 +536,7 @@ int qsv_batch_create(const int *nq, const int *layers, int64_t count, const doub
     for (int64_t i = 0; i < count; i++) {
         const int n = nq[i], L = layers[i];
         if (n < 2 || n > kMaxBatchQubits || L < 0) return fail(QSV_ERR_ARG, "instance qubits must be 2..22");
-        inst[i] = {n, L, (int64_t)mats.size(), i};
+        inst[i] = {n, L, (int64_t)mats.size(), i, 1};
         for (int l = 0; l < L; l++)
             for (int q = 0; q < n; q++) {
                 const double th = angles[aoff++], ph = angles[aoff++];
@@ -428,11 +563,23 @@ int qsv_batch_create(const int *nq, const int *layers, int64_t count, const doub
     if (hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && clk_khz > 0)
         b->clock_mhz = clk_khz / 1000.0;
     b->grid = cus;                            // 128 KiB of LDS per workgroup: one per CU
+    // split the largest instances: 2^(n - 19) parts from 20 qubits on, each
+    // with 32 tiles of every pass
+    std::vector<Item> items;
+    for (int64_t i = 0; i < count; i++) {
+        Inst &I = order[i];
+        if (I.n >= kSplitMinQubits && I.layers > 0)
+            I.parts = std::min({1 << (I.n - kSplitMinQubits + 1), kMaxParts, std::max(1, cus)});
+        for (int p = 0; p < I.parts; p++) items.push_back({(int)i, p});
+    }
+    b->nitems = (int64_t)items.size();
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc((void **)&b->d_inst, sizeof(Inst) * count) == hipSuccess &&
               hipMalloc((void **)&b->d_mats, sizeof(float) * std::max<size_t>(1, mats.size())) == hipSuccess &&
               hipMalloc((void **)&b->d_scratch, sizeof(float2) * ((size_t)b->grid << kMaxBatchQubits)) == hipSuccess &&
-              hipMalloc((void **)&b->d_counter, sizeof(unsigned int)) == hipSuccess &&
+              hipMalloc((void **)&b->d_items, sizeof(Item) * items.size()) == hipSuccess &&
+              hipMalloc((void **)&b->d_counter, sizeof(unsigned int) * (2 + 2 * (size_t)count)) == hipSuccess &&
+              hipMalloc((void **)&b->d_partial, sizeof(double) * kMaxParts * (size_t)count) == hipSuccess &&
               hipMalloc((void **)&b->d_zz, sizeof(double) * count) == hipSuccess &&
               hipMalloc((void **)&b->d_t0, sizeof(uint64_t) * count) == hipSuccess &&
               hipMalloc((void **)&b->d_t1, sizeof(uint64_t) * count) == hipSuccess;
@@ -441,6 +588,7 @@ int qsv_batch_create(const int *nq, const int *layers, int64_t count, const doub
         return fail(QSV_ERR_ALLOC, "batch allocation");
     }
     if (hipMemcpy(b->d_inst, order.data(), sizeof(Inst) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_items, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice) != hipSuccess ||
         (!mats.empty() && hipMemcpy(b->d_mats, mats.data(), sizeof(float) * mats.size(), hipMemcpyHostToDevice) != hipSuccess)) {
         batch_free(b);
         return fail(QSV_ERR_DEVICE, "batch upload");
@@ -465,6 +613,7 @@ int qsv_batch_autotune(qsv_batch *b, int *best_threads, double *best_ms)
         double ms = 0, m2 = 0;
         int rc = batch_timed(b, t, &ms);       // warm
         if (!rc) rc = batch_timed(b, t, &m2);
+        if (!rc) rc = batch_check(b);
         if (rc) return rc;
         if (m2 < best) {
             best = m2;

@@ -97,3 +97,22 @@ def test_two_pass_path_at_the_largest_sizes():
         say("threads", t, "sizes", nq)
         for n, a, z in zip(nq, ang, zz):
             assert abs(z - oracle_zz(n, a)) < 2e-4, (t, n)
+
+
+def test_split_instances_match_oracle_and_replay_exactly():
+    """Instances of 20, 21 and 22 qubits run as 2, 4 and 8 cooperating
+    workgroups (a per-instance barrier after every pass, per-part ZZ sums added
+    in part order): oracle values, and bit-identical eager and graph results."""
+    nq = [22, 21, 20, 22, 12, 17]
+    ly = [3, 2, 1, 1, 4, 2]
+    g = np.random.default_rng(77)
+    ang = [g.uniform(-np.pi, np.pi, (l, n, 2)) for n, l in zip(nq, ly)]
+    b = E.Batch(nq, ly, ang)
+    zz, lat, _ = b.run(512)                 # the graph's thread count (untuned: 512)
+    say("split sizes", nq, "latency us", lat)
+    for n, a, z in zip(nq, ang, zz):
+        assert abs(z - oracle_zz(n, a)) < 2e-4, n
+    for _ in range(2):
+        zg, _ = b.graph_run(replays=2)
+        assert np.array_equal(zg, zz)
+        assert np.array_equal(b.run(512)[0], zz)
