@@ -111,7 +111,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--cpu-stride", type=int, default=97)
     p.add_argument("--probe", action="store_true",
-                   help="counter probe: one state copy (known bytes) and three RX sweeps at n qubits")
+                   help="counter probe: read-only, write-only and copy passes (known bytes) and three RX sweeps")
     a = p.parse_args()
     a.split = {"fp32": 0, "split": 1, "bf16x6": 2, "fp16x3": 3}[a.product]
     if a.n is None:
@@ -122,20 +122,39 @@ def parse():
 
 
 def probe(a):
-    """Short fixed launch sequence for hardware-counter runs: the copy kernel
-    moves exactly 2 * 16 * 2^n bytes, which scales FETCH_SIZE / WRITE_SIZE;
-    then RX sweeps on qubits 0, n/2 and n-1 (same algorithmic bytes)."""
+    """Short fixed launch sequence for hardware-counter runs and the device's
+    streaming ceiling: the copy kernel moves exactly 2 * 16 * 2^n bytes, which
+    scales FETCH_SIZE / WRITE_SIZE; a read-only pass (the FP64 norm), a
+    write-only pass (|0...0>) and a copy, three of each, timed on the host
+    around a synchronised launch; then RX sweeps on qubits 0, n/2 and n-1 (same
+    algorithmic bytes as the copy)."""
     from qpanda_amd import circuits as C
     from qpanda_amd import engine as E
-    out = {"n": a.n, "copy_bytes": 2 * 16 * (1 << a.n)}
+    S = 16 * (1 << a.n)
+    out = {"n": a.n, "copy_bytes": 2 * S}
     with E.StateVector(a.n, E.C128) as s1, E.StateVector(a.n, E.C128) as s2:
         s2.copy_from(s1)
+        rates = {}
+        for name, fn, nbytes in (("read_only_norm", s1.norm2, S), ("write_only_fill", s1.reset, S),
+                                 ("copy", lambda: s2.copy_from(s1), 2 * S)):
+            best = None
+            for _ in range(3):
+                s1.sync()
+                s2.sync()
+                t0 = time.perf_counter()
+                fn()
+                s1.sync()
+                s2.sync()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            rates[name] = {"ms": best * 1e3, "gbs": nbytes / best / 1e9}
+        out["streaming"] = rates
         b = C._Builder(a.n)
         for q in (0, a.n // 2, a.n - 1):
             b.add(C.RX, q, theta=0.5)
         st = s1.run(b.build(), profile=True)
         out["rx_ms"] = st.class_ms[0] / max(1, st.class_launches[0])
-        out["rx_algorithmic_bytes"] = 2 * 16 * (1 << a.n)
+        out["rx_algorithmic_bytes"] = 2 * S
     print(json.dumps(out), flush=True)
 
 

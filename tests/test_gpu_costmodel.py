@@ -164,28 +164,33 @@ def test_trained_model_gain_over_greedy(split):
 
 @pytest.mark.skipif(not os.path.exists(E.DEFAULT_COSTMODEL), reason="no trained cost model in the tree")
 def test_trained_model_against_the_heuristic_schedulers():
-    """The learned scheduler chooses between block-first plans (k = 2..5) and
-    window plans (k = 4, 5) and how to launch them; on a held-out 28-qubit
-    random circuit it must beat greedy maximal grouping of k = 4 blocks (by
-    ~6% on this depth-24 circuit when the model was trained; ~1.4x at depth
-    200) and stay within 5% of the window planner's one-sweep-per-window
-    schedule."""
+    """Held-out 28-qubit circuits, five repetitions round-robin over the
+    methods (device drift hits every method alike; run-to-run spread of the
+    medians < 1%).  Random circuit: the learned schedule beats greedy maximal
+    grouping of k = 4 blocks and is the window planner's schedule up to one
+    tile-size choice (within 2%).  Hardware-efficient ansatz: the window
+    planner puts 16 blocks in its first sweep; the model splits that window
+    into two sweeps of 12 and 13 tile bits and folds the lone block after it
+    into the second (9.4 ms against 10.0 ms when trained), so it must be at
+    least 3% faster than the window heuristic there."""
     m = E.CostModel.load()
-    circ = C.random_layers(28, 24, 5151, haar_frac=0.1)
     methods = (("passes", dict(scheduler=2, group=8)), ("window", dict(scheduler=3)),
                ("learned", dict(scheduler=1)))
-    ms = {name: [] for name, _ in methods}
-    with E.StateVector(28, E.C64) as sv:
-        sv.attach_costmodel(m)
-        for _ in range(3):                   # round-robin: device drift hits every method alike
-            for name, kw in methods:
-                sv.reset()
-                st = sv.run(circ, fuse_k=4, split=3, **kw)
-                ms[name].append(st.device_ms)
-                if name == "learned":
-                    planner = st.planner
-    t = {name: float(np.median(v)) for name, v in ms.items()}
-    say("28q random: passes %.1f ms, window %.1f ms, learned %.1f ms (planner %d)"
-        % (t["passes"], t["window"], t["learned"], planner))
-    assert t["learned"] < t["passes"]
-    assert t["learned"] <= t["window"] * 1.05
+    res = {}
+    for circ in (C.random_layers(28, 24, 5151, haar_frac=0.1), C.hea(28, 6, 77)):
+        ms = {name: [] for name, _ in methods}
+        with E.StateVector(28, E.C64) as sv:
+            sv.attach_costmodel(m)
+            for _ in range(5):
+                for name, kw in methods:
+                    sv.reset()
+                    st = sv.run(circ, fuse_k=4, split=3, **kw)
+                    ms[name].append(st.device_ms)
+        t = {name: float(np.median(v)) for name, v in ms.items()}
+        say("%s: passes %.2f ms, window %.2f ms, learned %.2f ms" % (circ.name, t["passes"], t["window"], t["learned"]))
+        res[circ.name] = t
+    r = res["random_n28_d24_s5151"]
+    assert r["learned"] < r["passes"]
+    assert r["learned"] <= r["window"] * 1.02
+    h = res["hea_n28_l6_s77"]
+    assert h["learned"] * 1.03 <= h["window"]
