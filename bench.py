@@ -24,7 +24,8 @@ is computed on the device in FP64.
 
 Other workloads (one JSON line each; not the default line):
   --workload large33  33-qubit random circuit of depth 60, complex64 (64 GiB),
-                      learned schedule, FP64 norm, 10^5 measurement shots from
+                      learned schedule (and one run of k = 5 blocks in window
+                      sweeps for comparison), FP64 norm, 10^5 measurement shots from
                       the final state (linear cross-entropy of the samples), and
                       the fidelity of the same pipeline against the FP64 path on
                       the 28-qubit restriction of the same seeded generator;
@@ -496,6 +497,17 @@ def main_large(a):
     result = {}
     with E.StateVector(a.n, E.C64) as sv:
         sv.attach_costmodel(cm)
+        # comparison leg: k = 5 blocks in window sweeps (the learned schedule
+        # picks its own k; configuration 4 names k = 5)
+        log("k=5 window sweeps: one run (comparison)")
+        sv.reset()
+        k5 = sv.run(circ, fuse_k=5, split=a.split, scheduler=3, profile=True)
+        cl5 = k5.classes()
+        result["k5_windows"] = {"device_ms": k5.device_ms, "gate_apps_per_s_device": G / (k5.device_ms / 1e3),
+                                "blocks": int(k5.blocks),
+                                "sweeps": int(sum(cl5.get(c, {}).get("launches", 0) for c in ("fused", "pass"))),
+                                "ms_per_sweep": cl5.get("pass", {}).get("ms", 0.0) /
+                                max(1, cl5.get("pass", {}).get("launches", 0))}
         for _ in range(warmup):
             sv.reset()
             sv.run(circ, fuse_k=5, split=a.split, scheduler=1)
