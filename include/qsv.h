@@ -163,7 +163,7 @@ int qsv_sync(qsv_state *s);
  * with the same arithmetic, so the state is bit-identical to the
  * one-sweep-per-gate result.  Gates wider than a tile, dense blocks of 3+
  * qubits and states smaller than a tile sweep gate by gate.  Off by default;
- * the first call with on != 0 allocates 96 KiB of device staging.  A caller
+ * the first call with on != 0 allocates 208 KiB of device staging.  A caller
  * that writes the amplitude array through qsv_device_ptr must call qsv_sync
  * (or any readout) before each such write while tiling is on.  The reference
  * sweeps the state once per gate (CPUQuantumGates.cpp:92-1435,
