@@ -48,7 +48,7 @@ constexpr int kMaxBatchQubits = 22;
 
 constexpr int kSplitMinQubits = 20;          // instances from here on run as several parts
 constexpr int kMaxParts = 8;
-constexpr unsigned kSpinLimit = 1u << 26;    // barrier polls before the error flag (seconds)
+constexpr unsigned kSpinLimit = 1u << 22;    // barrier polls before the error flag (~1-10 s)
 
 struct Inst {
     int n, layers;
