@@ -156,8 +156,9 @@ int qsv_sync(qsv_state *s);
 
 /* Gate tiling (on != 0): the gate entry points below (and the per-gate path
  * of qsv_run_circuit) queue their sweeps instead of launching them, as long as
- * the queued gates' qubits fit one 64 KiB tile together with qubits 0..3 (12
- * qubits for complex128, 13 for complex64); the next call that is not a gate
+ * the queued gates' qubits fit one 64 KiB tile together with the qubits of a
+ * 128-byte run (12 qubits for complex128 with qubits 0..2, 13 for complex64
+ * with qubits 0..3); the next call that is not a gate
  * (any readout, measurement, copy, qsv_sync, qsv_device_ptr, qsv_set_gate_tiling
  * with on = 0, ...) applies the queue in one sweep over the state's tiles,
  * with the same arithmetic, so the state is bit-identical to the

@@ -1,8 +1,9 @@
 // qsv_gatemath.h -- the per-gate amplitude arithmetic, shared by the per-gate
 // sweeps (qsv_gates.hip) and the LDS-tiled multi-gate sweep (qsv_tiles.hip),
-// so that both compute bit-identical results: every function evaluates its
-// products and sums as written, with no fused multiply-add contraction, so
-// the result cannot depend on how the surrounding kernel was compiled.
+// so that both compute bit-identical results: every function spells out its
+// fused multiply-adds (and evaluates the rest as written, with no
+// contraction), so the result cannot depend on how the surrounding kernel
+// was compiled.
 //   pair ops  (H, X, 2x2, diagonal) on the two members of a pair;
 //   phase     one amplitude times a complex factor (CR, CZ);
 //   block     a dense 2^k x 2^k FP64 matrix on a group of 2^k amplitudes.
@@ -26,13 +27,28 @@ template <typename R> struct Coef {
     R u[8];
 };
 
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// acc + u x for complex u = ur + i ui: two fused multiply-adds per component
 template <typename T>
 __device__ __forceinline__ T cmadd(T acc, typename Real<T>::type ur, typename Real<T>::type ui, T x)
 {
-#pragma clang fp contract(off)
-    acc.x += ur * x.x - ui * x.y;
-    acc.y += ur * x.y + ui * x.x;
-    return acc;
+    T y;
+    y.x = fmadd(ur, x.x, fmadd(-ui, x.y, acc.x));
+    y.y = fmadd(ur, x.y, fmadd(ui, x.x, acc.y));
+    return y;
+}
+
+// u x for complex u = re + i im: one product and one fused multiply-add per
+// component
+template <typename T>
+__device__ __forceinline__ T cmul(T x, typename Real<T>::type re, typename Real<T>::type im)
+{
+    T y;
+    y.x = fmadd(re, x.x, -(im * x.y));
+    y.y = fmadd(re, x.y, im * x.x);
+    return y;
 }
 
 // (x0, x1) -> (y0, y1) for one pair; c: OP_H c.u[0] = the Hadamard constant,
@@ -49,8 +65,8 @@ __device__ __forceinline__ void pair_math(T x0, T x1, const Coef<typename Real<T
     } else if (OP == OP_X) {
         y0 = x1; y1 = x0;
     } else if (OP == OP_DIAG) {
-        y0.x = c.u[0] * x0.x - c.u[1] * x0.y; y0.y = c.u[0] * x0.y + c.u[1] * x0.x;
-        y1.x = c.u[2] * x1.x - c.u[3] * x1.y; y1.y = c.u[2] * x1.y + c.u[3] * x1.x;
+        y0 = cmul(x0, c.u[0], c.u[1]);
+        y1 = cmul(x1, c.u[2], c.u[3]);
     } else {
         y0.x = 0; y0.y = 0; y1.x = 0; y1.y = 0;
         y0 = cmadd(y0, c.u[0], c.u[1], x0); y0 = cmadd(y0, c.u[2], c.u[3], x1);
@@ -61,11 +77,7 @@ __device__ __forceinline__ void pair_math(T x0, T x1, const Coef<typename Real<T
 template <typename T>
 __device__ __forceinline__ T phase_math(T x, typename Real<T>::type re, typename Real<T>::type im)
 {
-#pragma clang fp contract(off)
-    T y;
-    y.x = re * x.x - im * x.y;
-    y.y = re * x.y + im * x.x;
-    return y;
+    return cmul(x, re, im);
 }
 
 // Row r of a dense d x d FP64 block (u: row-major complex doubles) applied to
@@ -80,8 +92,8 @@ __device__ __forceinline__ T block_row_math(const double *u, int d, int r, const
     for (int c = 0; c < MAXD; c++) {
         if (c < d) {
             const double ur = u[2 * (r * d + c)], ui = u[2 * (r * d + c) + 1];
-            sr += ur * xr[c] - ui * xi[c];
-            si += ur * xi[c] + ui * xr[c];
+            sr = fmadd(ur, xr[c], fmadd(-ui, xi[c], sr));
+            si = fmadd(ur, xi[c], fmadd(ui, xr[c], si));
         }
     }
     T y;

@@ -3,11 +3,12 @@
 //
 // With gate tiling on (qsv_set_gate_tiling), the gate entry points do not
 // launch their sweep: the pair, phase and two-qubit-block launches are queued
-// as long as every queued gate's qubits fit one tile together with qubits
-// 0..3 (2^12 amplitudes for complex128, 2^13 for complex64: 64 KiB of LDS).
+// as long as every queued gate's qubits fit one tile together with the
+// lowest qubits that make a 128-byte run (0..2 for complex128, 0..3 for
+// complex64; tiles of 2^12 or 2^13 amplitudes: 64 KiB of LDS).
 // The next entry point that is not a gate (a readout, a measurement, a state
 // copy, the end of a circuit) flushes the queue as one sweep over the state's
-// tiles: each workgroup loads a tile (runs of >= 16 contiguous amplitudes),
+// tiles: each workgroup loads a tile (128-byte runs of amplitudes),
 // applies the queued gates to it in program order with the per-gate kernels'
 // own arithmetic (qsv_gatemath.h, so the state is bit-identical to the
 // one-sweep-per-gate path), and stores it.  A gate whose qubits cannot join
@@ -71,8 +72,11 @@ namespace {
 // 64 KiB tiles: 2^12 complex128 or 2^13 complex64 amplitudes
 int tile_bits(const qsv_state *s) { return s->prec == QSV_C128 ? 12 : 13; }
 
-// qubits 0..3 and the gates' qubits must fit one tile together
-bool fits(const qsv_state *s, uint64_t mask) { return __builtin_popcountll(mask | uint64_t(0xF)) <= tile_bits(s); }
+// the qubits of one 128-byte run of amplitudes: every tile holds them
+uint64_t run_mask(const qsv_state *s) { return s->prec == QSV_C128 ? 0x7 : 0xF; }
+
+// the run's qubits and the gates' qubits must fit one tile together
+bool fits(const qsv_state *s, uint64_t mask) { return __builtin_popcountll(mask | run_mask(s)) <= tile_bits(s); }
 
 __device__ __forceinline__ uint32_t tile_expand(uint32_t p, const TileOp &op)
 {
@@ -187,7 +191,7 @@ int flush_now(qsv_state *s)
         rc = launch_direct(s, q->ops[0]);
     } else {
         const int tb = tile_bits(s);
-        uint64_t lmask = q->qmask | 0xF;
+        uint64_t lmask = q->qmask | run_mask(s);
         for (int b = 0; __builtin_popcountll(lmask) < tb; b++) lmask |= uint64_t(1) << b;
         TileArgs ta{};
         ta.tile_ins = insert_set_from_mask(lmask);
