@@ -137,7 +137,9 @@ public:
                     best = len;
                     tok.t = t;
                     const std::string lit = s_.substr(p_, len);
-                    if (t == T_INT) tok.ival = std::strtol(lit.c_str(), nullptr, 10);
+                    // rule 8 reads integers with atoi, (int) strtol: beyond INT_MAX the
+                    // value keeps its low 32 bits, as there
+                    if (t == T_INT) tok.ival = (int)std::strtol(lit.c_str(), nullptr, 10);
                     else tok.dval = std::strtod(lit.c_str(), nullptr);
                 }
             }
@@ -146,7 +148,7 @@ public:
                 tok.t = T_CHAR;
                 tok.ch = c;
             }
-            if (best == 0) {                      // rule 39: any other character is skipped
+            if (best == 0) {                      // rule 40: any other character is echoed, no token
                 p_++;
                 continue;
             }
@@ -251,9 +253,13 @@ private:
         advance();
         return true;
     }
+    // an integer in an index position (QINIT, CREG, qubits, classical bits);
+    // one that wrapped to a negative value is refused, where the reference
+    // would go on to use it as an index
     bool expectInt(long *v)
     {
         if (cur_.t != T_INT) return fail("syntax error, expected an integer");
+        if (cur_.ival < 0) return fail("integer out of range");
         *v = cur_.ival;
         advance();
         return true;

@@ -11,10 +11,9 @@ Pins:
     (LinkedListController.cpp:51-339) by hand: DAGGER reversal and daggered
     flags by nesting parity, CONTROL lists, dropped measurements, the angle
     arithmetic including "a - b" = a + b and left-associative '^'.
-The C++ parser was also compared, accept/reject and instruction list, with a
-walk of the reference lexer's and parser's own tables on 7,000 generated
-programs (0 differences); that comparison needs the reference tree and is not
-part of this suite."""
+tests/test_qrunes_tables.py compares the same parser, accept / reject and
+instruction list, with a walk of the reference lexer's and parser's own
+tables."""
 import math
 import os
 
