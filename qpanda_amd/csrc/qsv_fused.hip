@@ -240,8 +240,8 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int m = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);   // real output row
-            const int e = fa.erow[m >> 1] | ecol;
-            if (m & 1) pim[e] = acc[h][v];
+            const int e = fa.erow[out_row(m)] | ecol;
+            if (out_plane(m)) pim[e] = acc[h][v];
             else pre[e] = acc[h][v];
         }
     __syncthreads();
@@ -288,10 +288,12 @@ static double from_bf16(uint16_t b)
 void fused_operands(int kq, const double *u, int split, float *out)
 {
     const int d = 1 << kq, R = 2 * d, halves = R / 32;
+    // real row i: complex row out_row(i), plane out_plane(i); real column j:
+    // complex column j >> 1, plane j & 1 (the K order of the B operand)
     auto M = [&](int i, int j) -> double {
-        const int r = i >> 1, c = j >> 1;
+        const int r = out_row(i), c = j >> 1;
         const double ur = u[2 * (r * d + c)], ui = u[2 * (r * d + c) + 1];
-        if ((i & 1) == 0) return (j & 1) == 0 ? ur : -ui;
+        if (out_plane(i) == 0) return (j & 1) == 0 ? ur : -ui;
         return (j & 1) == 0 ? ui : ur;
     };
     if (split == 3) {

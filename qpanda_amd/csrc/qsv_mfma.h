@@ -26,6 +26,16 @@
 namespace qsv {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Output rows of the real product.  Real row m of the 2^(k+1) x 2^(k+1) real
+// matrix holds complex row out_row(m), plane out_plane(m) (0 = re, 1 = im):
+// m = 8 a + 4 b + 2 c + d -> row 4 a + 2 b + d, plane c.  The 32x32 MFMA
+// leaves rows 8 (v >> 2) + 4 hf + (v & 3) in accumulator register v of lane
+// half hf, so registers (v, v + 1), v even, are two adjacent complex rows of
+// one plane: an 8-byte pair of the planar LDS tile when the rows differ in
+// tile bit 0, written without register moves.
+__host__ __device__ constexpr int out_row(int m) { return 4 * (m >> 3) + 2 * ((m >> 2) & 1) + (m & 1); }
+__host__ __device__ constexpr int out_plane(int m) { return (m >> 1) & 1; }
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x2v __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x2v __attribute__((ext_vector_type(2)));
@@ -34,11 +44,12 @@ typedef _Float16 f16x2v __attribute__((ext_vector_type(2)));
 // operand in the permuted K order of the BF16 products (K-block sb, lane half
 // hf, element e <-> real column 2 (8 sb + e) + hf), so the B values of K-block
 // sb are this lane's FP32 B values bv[8 sb .. 8 sb + 7], already scaled.
-// `one` must be 1.0f but unknown to the compiler (a kernel argument): the
-// residual x * one - fp16(x), rounded to fp16, then compiles to one
-// v_fma_mixlo_f16 / v_fma_mixhi_f16 that reads the fp16 half in place and
-// writes the low piece's half in place, where x - fp16(x) with a literal
-// costs a conversion and a subtraction.
+// `one` must be 1.0f but unknown to the compiler (a kernel argument), so the
+// residual x * one - fp16(x) is formed by fma-mix instructions that read the
+// fp16 half in place (with a literal 1 it costs a conversion and a
+// subtraction).  Written-out v_fma_mixlo/hi_f16 (one instruction per value
+// instead of the compiler's mix-to-FP32 plus a packing conversion) measured
+// 1% slower per window sweep: the VALU count is not what bounds the blocks.
 template <int HALVES, int LOBLK>
 __device__ __forceinline__ void fp16x3_product(floatx16 (&acc)[HALVES], const f16x8 (&ap)[2][HALVES][LOBLK],
                                                const float *bv, float one)
@@ -52,8 +63,7 @@ __device__ __forceinline__ void fp16x3_product(floatx16 (&acc)[HALVES], const f1
             const f16x2v h = __builtin_convertvector((f32x2v){x0, x1}, f16x2v);
             b0[e] = h[0];
             b0[e + 1] = h[1];
-            // the residual is exact in FP32; fma-mix rounds it to fp16 directly
-            // into either half of the register (v_fma_mixlo/hi_f16)
+            // the residual is exact in FP32 and rounds to fp16 (v_fma_mix*)
             b1[e] = (_Float16)__builtin_fmaf(x0, one, -(float)h[0]);
             b1[e + 1] = (_Float16)__builtin_fmaf(x1, one, -(float)h[1]);
         }

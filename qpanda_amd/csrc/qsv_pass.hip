@@ -30,8 +30,14 @@
 // that prefetches the next tile into registers measured ~10% slower at 30
 // qubits than one workgroup per tile with the FP32 product, and 4% slower with
 // BF16x6 on 64 KiB tiles, where it costs no occupancy; capping registers for
-// five resident 32 KiB tiles per CU measured 4% slower as well.)  Whether a grouping pays is a
-// scheduling decision (greedy grouping, or the cost model in qsv_costmodel.hip).
+// five resident 32 KiB tiles per CU measured 4% slower as well.  On the 30-qubit
+// benchmark plan with FP16x3, 4.21 ms per window sweep here against: one
+// persistent 1024-thread workgroup per CU with the next tile prefetched into
+// registers and the operands in LDS, 4.79 ms (bit-identical; without blocks it
+// streams at 3.65 ms against 3.46 ms for this form); two persistent
+// 512-thread workgroups per CU with the same prefetch, 4.49 ms.)  Whether a
+// grouping pays is a scheduling decision (greedy grouping, or the cost model in
+// qsv_costmodel.hip).
 #include "qsv_internal.h"
 #include "qsv_mfma.h"
 #include "qsv_sched.h"
@@ -216,8 +222,8 @@ __device__ __forceinline__ void pass_apply(float *pre, float *pim, const PassBlo
 #pragma unroll
             for (int v = 0; v < 16; v++) {
                 const int m = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const int e = pb.erow[m >> 1] ^ ecol;
-                if (m & 1) pim[e] = acc[h][v];
+                const int e = pb.erow[out_row(m)] ^ ecol;
+                if (out_plane(m)) pim[e] = acc[h][v];
                 else pre[e] = acc[h][v];
             }
     }
@@ -337,13 +343,14 @@ __device__ __forceinline__ void pass_apply4(float *pre, const PassBlock &pb, con
             }
         }
         // output register v holds real row m = 8 (v >> 2) + 4 hf + (v & 3):
-        // complex row 4 (v >> 2) + 2 hf + ((v & 3) >> 1), plane v & 1; the
-        // plane bit of ecol belongs to the B reads, so it is cleared here
+        // complex row 4 (v >> 2) + 2 hf + (v & 1), plane (v >> 1) & 1 (out_row,
+        // out_plane); the plane bit of ecol belongs to the B reads, so it is
+        // cleared here
         const int wcol = ecol & ~(1 << (TB + 2));
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int j = 2 * (v >> 2) + ((v & 3) >> 1);
-            *reinterpret_cast<float *>(base + (wa0[j] ^ wdm[j] ^ wcol) + (v & 1) * 4 * PLANE) = acc[v];
+            const int j = 2 * (v >> 2) + (v & 1);
+            *reinterpret_cast<float *>(base + (wa0[j] ^ wdm[j] ^ wcol) + ((v >> 1) & 1) * 4 * PLANE) = acc[v];
         }
     }
 }
@@ -401,15 +408,15 @@ __device__ __forceinline__ void pass_apply4_rows(float *pre, const PassBlock &pb
         for (int v = 0; v < 16; v++) acc[0][v] = 0.0f;
         if constexpr (MODE == 2) pass_bf16x6<1, 2>(acc, o.p, bv);
         else fp16x3_product<1, 2>(acc, o.h, bv, one);
-        // acc[4 j + u]: row 4 j + 2 hf + (u >> 1), plane u & 1
+        // acc[4 j + u]: row 4 j + 2 hf + (u & 1), plane u >> 1 (out_row,
+        // out_plane), so each register pair is one plane's 8-byte pair
         const int wcol = ecol & ~(1 << (TB + 2));
-        // 8-byte stores of both planes (ds_write2st64_b64; measured faster than
-        // dword stores that avoid the register moves the pairs need)
+        // both planes in one ds_write2st64_b64, straight from the accumulators
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             char *const a = base + (wa0[j] ^ wdm[j] ^ wcol);
-            *reinterpret_cast<float2 *>(a) = make_float2(acc[0][4 * j], acc[0][4 * j + 2]);
-            *reinterpret_cast<float2 *>(a + 4 * PLANE) = make_float2(acc[0][4 * j + 1], acc[0][4 * j + 3]);
+            *reinterpret_cast<float2 *>(a) = make_float2(acc[0][4 * j], acc[0][4 * j + 1]);
+            *reinterpret_cast<float2 *>(a + 4 * PLANE) = make_float2(acc[0][4 * j + 2], acc[0][4 * j + 3]);
         }
     }
 }
@@ -458,7 +465,7 @@ __device__ __forceinline__ void pass_apply4_cols(float *pre, const PassBlock &pb
         fp16x3_product<1, 2>(acc0, o.h, bv0, one);
         fp16x3_product<1, 2>(acc1, o.h, bv1, one);
     }
-    // output register v: row 4 (v >> 2) + 2 hf + ((v & 3) >> 1), plane v & 1
+    // output register v: row 4 (v >> 2) + 2 hf + (v & 1), plane (v >> 1) & 1
     const int wcol = ecol & ~(1 << (TB + 2));
     const int hfmask = -hf;
 #pragma unroll
@@ -466,9 +473,9 @@ __device__ __forceinline__ void pass_apply4_cols(float *pre, const PassBlock &pb
         const int r0 = 4 * (j >> 1) + (j & 1);
         const int a0 = 4 * pb.erow[r0], d = (a0 ^ (4 * pb.erow[r0 + 2])) & hfmask;
         char *const a = base + (a0 ^ d ^ wcol);
-        const int v = 4 * (j >> 1) + 2 * (j & 1);
+        const int v = 4 * (j >> 1) + (j & 1);
         *reinterpret_cast<float2 *>(a) = make_float2(acc0[0][v], acc1[0][v]);
-        *reinterpret_cast<float2 *>(a + 4 * PLANE) = make_float2(acc0[0][v + 1], acc1[0][v + 1]);
+        *reinterpret_cast<float2 *>(a + 4 * PLANE) = make_float2(acc0[0][v + 2], acc1[0][v + 2]);
     }
 }
 
