@@ -129,6 +129,7 @@ int qsv_destroy(qsv_state *s)
     if (!s) return QSV_OK;
     tiles_free(s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_operand_ring(s);
     if (s->amps) (void)hipFree(s->amps);
     if (s->red) (void)hipFree(s->red);
     if (s->aux) (void)hipFree(s->aux);

@@ -30,6 +30,12 @@ struct qsv_costmodel {
     hipStream_t stream = nullptr;
     int64_t step = 0;
     std::mutex mu;               // one call at a time (the scheduler scores candidates on threads)
+    // inference buffers, grown as needed and kept: the scheduler predicts
+    // while the state's stream runs sweeps, and freeing device memory would
+    // wait for them
+    float *ix = nullptr, *iy = nullptr, *iact = nullptr;   // device
+    float *hx = nullptr, *hy = nullptr;                     // pinned host staging
+    int64_t icap = 0;
 };
 
 namespace qsv {
@@ -133,6 +139,17 @@ struct Batch {
             if (p) (void)hipFree(p);
     }
 };
+
+void free_inference(qsv_costmodel *m)
+{
+    if (m->ix) (void)hipFree(m->ix);
+    if (m->iy) (void)hipFree(m->iy);
+    if (m->iact) (void)hipFree(m->iact);
+    if (m->hx) (void)hipHostFree(m->hx);
+    if (m->hy) (void)hipHostFree(m->hy);
+    m->ix = m->iy = m->iact = m->hx = m->hy = nullptr;
+    m->icap = 0;
+}
 
 int upload_batch(qsv_costmodel *m, const float *x, const float *t, int64_t batch, bool need_grad, Batch &B)
 {
@@ -561,6 +578,7 @@ int qsv_cm_destroy(qsv_costmodel *m)
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (float *p : {m->params, m->m, m->v, m->grad})
         if (p) (void)hipFree(p);
+    free_inference(m);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return QSV_OK;
@@ -590,12 +608,27 @@ int qsv_cm_forward(qsv_costmodel *m, const float *x, int64_t batch, float *y)
     if (!m || !x || !y || batch < 0) return fail(QSV_ERR_ARG, "bad argument");
     if (batch == 0) return QSV_OK;
     std::lock_guard<std::mutex> lock(m->mu);
-    Batch B;
-    int rc = upload_batch(m, x, nullptr, batch, false, B);
-    if (!rc) rc = forward(m, B);
-    if (rc) return rc;
-    QSV_HIP(hipMemcpyAsync(y, B.y, sizeof(float) * batch, hipMemcpyDeviceToHost, m->stream));
+    if (batch > m->icap) {
+        free_inference(m);
+        const int64_t cap = std::max<int64_t>(batch, 65536);
+        if (hipMalloc((void **)&m->ix, sizeof(float) * F * cap) != hipSuccess ||
+            hipMalloc((void **)&m->iy, sizeof(float) * cap) != hipSuccess ||
+            hipMalloc((void **)&m->iact, sizeof(float) * 2 * H * cap) != hipSuccess ||
+            hipHostMalloc((void **)&m->hx, sizeof(float) * F * cap, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&m->hy, sizeof(float) * cap, hipHostMallocDefault) != hipSuccess) {
+            free_inference(m);
+            return fail(QSV_ERR_ALLOC, "cost-model inference buffers");
+        }
+        m->icap = cap;
+    }
+    std::memcpy(m->hx, x, sizeof(float) * F * batch);
+    QSV_HIP(hipMemcpyAsync(m->ix, m->hx, sizeof(float) * F * batch, hipMemcpyHostToDevice, m->stream));
+    const unsigned grid = (unsigned)((batch + 127) / 128);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(128), 0, m->stream, m->params, m->ix, batch, m->iy, m->iact);
+    QSV_HIP(hipGetLastError());
+    QSV_HIP(hipMemcpyAsync(m->hy, m->iy, sizeof(float) * batch, hipMemcpyDeviceToHost, m->stream));
     QSV_HIP(hipStreamSynchronize(m->stream));
+    std::memcpy(y, m->hy, sizeof(float) * batch);
     return QSV_OK;
 }
 

@@ -21,7 +21,12 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <limits>
+#include <mutex>
+#include <string>
 #include <thread>
 
 namespace qsv {
@@ -349,6 +354,12 @@ void move_qubit_first(PlanItem &it, int q)
     std::vector<int> order{j};                    // new row bit b <- old row bit order[b]
     for (int i = 0; i < k; i++)
         if (i != j) order.push_back(i);
+    std::vector<int> qs(k);
+    for (int b = 0; b < k; b++) qs[b] = it.qubits[order[b]];
+    if (it.u.empty()) {                           // a shape: qubits only
+        it.qubits.swap(qs);
+        return;
+    }
     const int d = 1 << k;
     auto old_index = [&](int x) {
         int o = 0;
@@ -363,8 +374,6 @@ void move_qubit_first(PlanItem &it, int q)
             u[2 * dst] = it.u[2 * src];
             u[2 * dst + 1] = it.u[2 * src + 1];
         }
-    std::vector<int> qs(k);
-    for (int b = 0; b < k; b++) qs[b] = it.qubits[order[b]];
     it.qubits.swap(qs);
     it.u.swap(u);
 }
@@ -377,6 +386,10 @@ void widen_with(PlanItem &it, const std::vector<int> &add)
     if (add.empty()) return;
     std::vector<int> qs = it.qubits;
     qs.insert(qs.end(), add.begin(), add.end());
+    if (it.u.empty()) {                           // a shape: qubits only
+        it.qubits = qs;
+        return;
+    }
     const int d = 1 << k, D = 1 << kq;
     std::vector<double> w(2 * (size_t)D * D, 0.0);
     for (int hi = 0; hi < D / d; hi++)
@@ -491,130 +504,99 @@ void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only)
     }
 }
 
-int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
-              const qsv_run_opts &o, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms,
-              Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used, int *planner)
+namespace {
+
+constexpr int kKqMin = 4;
+
+// Widening and kernel choice of a plan's blocks: window plans widen their
+// small blocks with the window's highest qubits (window_widening), the rest
+// widen to >= 4 qubits with the lowest free ones where an MFMA tile fits;
+// other blocks use the FP64 kernel.  mfma[i] = 1 for MFMA blocks.
+void prepare_blocks(const qsv_state *s, std::vector<PlanItem> &plan, const std::vector<int> &window,
+                    std::vector<char> &mfma)
 {
-    if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "the fused path runs on c64 states");
-    const auto t_plan = std::chrono::steady_clock::now();
-    if (o.tile != 0 && (o.tile < 6 || o.tile > 8)) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
-    std::vector<PlanItem> plan;
-    std::vector<int> window;
-    int rc;
-    SchedChoice choice;
-    const bool windowed = o.scheduler == 3 && s->n >= 11;
-    if (o.pass_tb != 0 && (o.pass_tb < 11 || o.pass_tb > kMaxPassTileBits))
-        return fail(QSV_ERR_ARG, "pass tile bits must be 0 or 11..13");
-    if (o.scheduler == 1) {
-        rc = plan_learned(s, gates, ngates, mats, o, plan, &choice, &window);
-    } else if (windowed) {
-        choice.fuse_k = std::max(4, std::min(o.fuse_k, 5));
-        choice.window = 1;
-        rc = plan_window(gates, ngates, mats, choice.fuse_k, o.exact, o.pass_tb ? o.pass_tb : kMaxPassTileBits,
-                         s->n, plan, window);
-    } else {
-        rc = plan_greedy(gates, ngates, mats, std::min(o.fuse_k, 5), o.exact, plan);
-        choice.fuse_k = o.fuse_k;
-    }
-    if (rc) return rc;
-    const int kq_min = 4;
-    // window plans widen their small blocks with the window's highest qubits
-    // (window_widening); the rest widen to >= 4 qubits with the lowest free
-    // ones where an MFMA tile fits; other blocks use the FP64 kernel
-    if (!window.empty() && s->n >= kq_min + 6)
+    if (!window.empty() && s->n >= kKqMin + 6)
         for (size_t i = 0; i < plan.size(); i++)
-            if (plan[i].barrier < 0) widen_with(plan[i], window_widening(plan, window, i, kq_min));
-    std::vector<char> mfma(plan.size(), 0);
+            if (plan[i].barrier < 0) widen_with(plan[i], window_widening(plan, window, i, kKqMin));
+    mfma.assign(plan.size(), 0);
     for (size_t i = 0; i < plan.size(); i++) {
         PlanItem &it = plan[i];
         if (it.barrier >= 0) continue;
-        const int kq = std::max(kq_min, (int)it.qubits.size());
+        const int kq = std::max(kKqMin, (int)it.qubits.size());
         if (s->n < kq + 6) continue;
-        widen_to(it, kq_min);
+        widen_to(it, kKqMin);
         move_qubit_first(it, 0);
         mfma[i] = 1;
     }
-    std::vector<Launch> launches;
-    double pred = 0;
-    if (o.scheduler == 1 && s->cm) {
-        rc = group_launches(s, plan, o, launches, &pred, window.empty() ? nullptr : &window);
-    } else if (windowed) {
-        // one sweep per window (several beyond the pass kernel's block limit)
-        const int tb_min = o.pass_tb ? o.pass_tb : 11;
-        for (size_t i = 0; i < plan.size();) {
-            size_t j = i + 1;
-            while (j < plan.size() && plan[i].barrier < 0 && plan[j].barrier < 0 && window[j] == window[i]) j++;
-            for (size_t b = i; b < j; b += kMaxPassBlocks) {
-                Launch L;
-                L.first = b;
-                L.count = (int)std::min<size_t>(kMaxPassBlocks, j - b);
-                if (plan[b].barrier >= 0) {
-                    L.count = 1;
-                } else if (L.count == 1) {
-                    const int kq = (int)plan[b].qubits.size();
-                    L.tc = mfma[b] ? std::max(6, std::min(kDefaultTileCols, s->n - kq)) : 0;
-                } else {
-                    std::vector<const std::vector<int> *> q;
-                    for (int g = 0; g < L.count; g++) q.push_back(&plan[b + g].qubits);
-                    L.tb = std::max(pass_tile_bits(q, s->n), std::min(tb_min, s->n));
-                }
-                launches.push_back(L);
+}
+
+// The window heuristic's launches: one sweep per window (several beyond the
+// pass kernel's block limit).
+void window_launches(const qsv_state *s, const std::vector<PlanItem> &plan, const std::vector<int> &window,
+                     const std::vector<char> &mfma, const qsv_run_opts &o, std::vector<Launch> &launches)
+{
+    const int tb_min = o.pass_tb ? o.pass_tb : 11;
+    for (size_t i = 0; i < plan.size();) {
+        size_t j = i + 1;
+        while (j < plan.size() && plan[i].barrier < 0 && plan[j].barrier < 0 && window[j] == window[i]) j++;
+        for (size_t b = i; b < j; b += kMaxPassBlocks) {
+            Launch L;
+            L.first = b;
+            L.count = (int)std::min<size_t>(kMaxPassBlocks, j - b);
+            if (plan[b].barrier >= 0) {
+                L.count = 1;
+            } else if (L.count == 1) {
+                const int kq = (int)plan[b].qubits.size();
+                L.tc = mfma[b] ? std::max(6, std::min(kDefaultTileCols, s->n - kq)) : 0;
+            } else {
+                std::vector<const std::vector<int> *> q;
+                for (int g = 0; g < L.count; g++) q.push_back(&plan[b + g].qubits);
+                L.tb = std::max(pass_tile_bits(q, s->n), std::min(tb_min, s->n));
             }
-            i = j;
+            launches.push_back(L);
         }
-    } else {
-        qsv_run_opts g = o;
-        if (o.scheduler != 2) g.group = 1;
-        rc = group_launches(s, plan, g, launches, &pred);
+        i = j;
     }
-    if (rc) return rc;
-    // operands for every MFMA block in one upload, built on a few threads
-    std::vector<size_t> op_off(plan.size(), 0);
-    size_t op_total = 0;
+}
+
+// Operands of every MFMA block (float offsets op_off[i] into ops), built on
+// up to `threads` threads.
+void build_operands(const std::vector<PlanItem> &plan, const std::vector<char> &mfma, int split, int threads,
+                    std::vector<size_t> &op_off, std::vector<float> &ops)
+{
+    op_off.assign(plan.size(), 0);
+    size_t total = 0;
     for (size_t i = 0; i < plan.size(); i++) {
         if (!mfma[i]) continue;
-        op_off[i] = op_total;
-        op_total += fused_operand_floats((int)plan[i].qubits.size(), o.split);
+        op_off[i] = total;
+        total += fused_operand_floats((int)plan[i].qubits.size(), split);
     }
-    std::vector<float> ops(op_total);
-    {
-        const size_t nth = std::max<size_t>(1, std::min<size_t>({8, std::thread::hardware_concurrency(),
-                                                                  plan.size() / 64 + 1}));
-        std::vector<std::thread> pool;
-        for (size_t t = 0; t < nth; t++)
-            pool.emplace_back([&, t]() {
-                for (size_t i = t; i < plan.size(); i += nth)
-                    if (mfma[i])
-                        fused_operands((int)plan[i].qubits.size(), plan[i].u.data(), o.split, ops.data() + op_off[i]);
-            });
-        for (std::thread &th : pool) th.join();
+    ops.assign(total, 0.0f);
+    const size_t nth = std::max<size_t>(1, std::min<size_t>({(size_t)threads, std::thread::hardware_concurrency(),
+                                                             plan.size() / 64 + 1}));
+    auto work = [&](size_t t) {
+        for (size_t i = t; i < plan.size(); i += nth)
+            if (mfma[i]) fused_operands((int)plan[i].qubits.size(), plan[i].u.data(), split, ops.data() + op_off[i]);
+    };
+    if (nth == 1) {
+        work(0);
+        return;
     }
-    float *dops = nullptr;
-    if (!ops.empty()) {
-        if (hipMalloc((void **)&dops, ops.size() * sizeof(float)) != hipSuccess)
-            return fail(QSV_ERR_ALLOC, "cannot allocate block operands");
-        hipError_t e = hipMemcpyAsync(dops, ops.data(), ops.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-        if (e != hipSuccess) {
-            (void)hipFree(dops);
-            return hip_fail(e, "operand upload");
-        }
-    }
-    if (o.split == 3 && (rc = f16_state_scale(s))) {
-        if (dops) (void)hipFree(dops);
-        return rc;
-    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nth; t++) pool.emplace_back(work, t);
+    for (std::thread &th : pool) th.join();
+}
+
+// Issues a plan's launches on the state's stream; operands of block i at
+// dops + op_off[i].  Barrier records (MEASURE / RESET) draw and measure on
+// the host's side of the stream.
+int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanItem> &plan,
+                  const std::vector<char> &mfma, const std::vector<Launch> &launches, const float *dops,
+                  const std::vector<size_t> &op_off, const qsv_run_opts &o, int32_t *rng, int *outcomes,
+                  Profiler *prof, int64_t *nblocks)
+{
     const double N = (double)s->dim;
-    int64_t nblocks = 0;
-    *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
-    *k_used = choice.fuse_k;
-    *planner = choice.window;
-    QSV_HIP(hipEventRecord(start, s->stream));       // device time starts after planning
-    rc = QSV_OK;
-    if (prof->on()) {
-        s->tlog_feats.clear();
-        s->tlog_ms.clear();
-    }
+    int rc = QSV_OK;
     for (const Launch &L : launches) {
         if (rc) break;
         const PlanItem &it0 = plan[L.first];
@@ -672,8 +654,421 @@ int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double 
             s->tlog_feats.insert(s->tlog_feats.end(), f, f + QSV_CM_FEATURES);
         }
         if (!rc) rc = prof->end(cls, 2 * 8 * N, flops, tag);
-        nblocks += L.count;
+        *nblocks += L.count;
     }
+    return rc;
+}
+
+// ---- streamed window schedules ---------------------------------------------
+//
+// The window schedules (the learned one and the window heuristic) are planned
+// and run as a pipeline: a planning thread takes the windows from a
+// WindowStream (for the learned schedule, the beam search), fuses each
+// window's records, lets the cost model choose the window's block size (4 or
+// 5) and launch split, and builds its operands; the calling thread copies
+// each window's operands through a pinned ring into a device ring and issues
+// its sweeps.  So all but the first window's planning overlaps the device's
+// work on earlier windows.  The operand copies are ordered on the state's
+// stream after every earlier sweep, so a ring slot is only overwritten once the
+// sweeps that read it have run; a host slot is reused once the event recorded
+// after its window's sweeps has completed.
+
+// k = 5 only on a predicted gain above the model's per-launch error (at 5%
+// the model picked k = 5 for windows that ran slower)
+constexpr double kK5Margin = 0.10;
+
+struct ReadyWindow {
+    int64_t barrier = -1;
+    bool end = false;
+    int rc = QSV_OK;
+    std::string err;
+    std::vector<PlanItem> plan;
+    std::vector<char> mfma;
+    std::vector<Launch> launches;
+    std::vector<size_t> op_off;
+    std::vector<float> ops;
+    int k = 4;
+    double pred = 0;
+};
+
+class WindowQueue {
+public:
+    explicit WindowQueue(size_t cap) : cap_(cap) {}
+    void push(ReadyWindow &&w)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return q_.size() < cap_ || stop_; });
+        if (stop_) return;
+        q_.push_back(std::move(w));
+        cv_.notify_all();
+    }
+    ReadyWindow pop()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !q_.empty(); });
+        ReadyWindow w = std::move(q_.front());
+        q_.pop_front();
+        cv_.notify_all();
+        return w;
+    }
+    void stop()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+        cv_.notify_all();
+    }
+    bool stopped()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        return stop_;
+    }
+
+private:
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<ReadyWindow> q_;
+    size_t cap_;
+    bool stop_ = false;
+};
+
+// One window's blocks at block size k (shapes or with matrices), widened and
+// with their launches: the cost model's DP for the learned schedule, one
+// sweep (split at the pass kernel's limit) for the heuristic.
+int plan_one_window(qsv_state *s, const qsv_gate *gates, const double *mats, const qsv_run_opts &o,
+                    const WindowUnit &u, int k, bool shapes, bool learned, std::vector<PlanItem> &plan,
+                    std::vector<char> &mfma, std::vector<Launch> &launches, double *pred)
+{
+    int rc = fuse_window(gates, u.taken, mats, k, o.exact, shapes, plan);
+    if (rc) return rc;
+    const std::vector<int> window(plan.size(), 0);
+    prepare_blocks(s, plan, window, mfma);
+    launches.clear();
+    *pred = 0;
+    if (learned) return group_launches(s, plan, o, launches, pred, &window);
+    window_launches(s, plan, window, mfma, o, launches);
+    return QSV_OK;
+}
+
+void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats, const qsv_run_opts &o,
+                     bool learned, WindowQueue *q)
+{
+    auto send_error = [&](int rc) {
+        ReadyWindow w;
+        w.rc = rc;
+        w.err = qsv_last_error();
+        q->push(std::move(w));
+    };
+    // a new thread's current device is device 0: plan on the state's
+    if (hipSetDevice(s->device) != hipSuccess) {
+        send_error(fail(QSV_ERR_DEVICE, "cannot select the device"));
+        return;
+    }
+    // the beam search costs host time per window (~0.5 ms at width 4, ~1.5 ms
+    // at width 12 on 30 qubits): it is sized to stay ahead of the device,
+    // whose window sweeps take ~2 ms at 29 qubits and ~4.2 ms at 30
+    BeamOpts bo;
+    if (learned && s->n >= 30) {
+        bo.width = 12;
+        bo.branch = 4;
+        bo.first = 8;
+        bo.delay = 6;
+    } else if (learned && s->n >= 29) {
+        bo.width = 4;
+        bo.branch = 3;
+        bo.first = 6;
+        bo.delay = 4;
+    }
+    const int fixed_k = std::max(4, std::min(o.fuse_k, 5));
+    WindowStream ws(gates, ngates, s->n, o.pass_tb ? o.pass_tb : kMaxPassTileBits, bo);
+    std::vector<PlanItem> shape, shape5;
+    std::vector<char> shape_mfma, mfma5;
+    std::vector<Launch> shape_launches, launches5;
+    while (!q->stopped()) {
+        WindowUnit u;
+        bool end = false;
+        int rc = ws.next(u, &end);
+        if (rc) return send_error(rc);
+        ReadyWindow w;
+        if (end) {
+            w.end = true;
+            q->push(std::move(w));
+            return;
+        }
+        if (u.barrier >= 0) {
+            w.barrier = u.barrier;
+            PlanItem it;
+            it.barrier = u.barrier;
+            w.plan.push_back(std::move(it));
+            w.mfma.assign(1, 0);
+            Launch L;
+            L.first = 0;
+            w.launches.push_back(L);
+            w.op_off.assign(1, 0);
+            q->push(std::move(w));
+            continue;
+        }
+        // the learned schedule predicts the window at k = 4 and k = 5 (as
+        // shapes) and takes k = 5 only for a predicted gain above the model's
+        // resolution; then the blocks get their matrices
+        int k = fixed_k;
+        double best = 0;
+        if (learned) {
+            double p4 = 0, p5 = 0;
+            if ((rc = plan_one_window(s, gates, mats, o, u, 5, true, true, shape5, mfma5, launches5, &p5)) ||
+                (rc = plan_one_window(s, gates, mats, o, u, 4, true, true, shape, shape_mfma, shape_launches, &p4)))
+                return send_error(rc);
+            k = 4;
+            best = p4;
+            if (p5 < p4 * (1 - kK5Margin)) {
+                k = 5;
+                best = p5;
+                shape.swap(shape5);
+                shape_launches.swap(launches5);
+            }
+        }
+        rc = fuse_window(gates, u.taken, mats, k, o.exact, false, w.plan);
+        if (rc) return send_error(rc);
+        const std::vector<int> window(w.plan.size(), 0);
+        prepare_blocks(s, w.plan, window, w.mfma);
+        if (learned) {
+            bool same = w.plan.size() == shape.size();
+            for (size_t i = 0; same && i < shape.size(); i++) same = w.plan[i].qubits == shape[i].qubits;
+            if (!same) return send_error(fail(QSV_ERR_UNSUPPORTED, "planner shapes differ with and without block matrices"));
+            w.launches = shape_launches;
+        } else {
+            window_launches(s, w.plan, window, w.mfma, o, w.launches);
+        }
+        w.k = k;
+        w.pred = best;
+        build_operands(w.plan, w.mfma, o.split, 1, w.op_off, w.ops);
+        q->push(std::move(w));
+    }
+}
+
+// Operand rings of the streamed schedules (kept with the state).
+struct OperandRing {
+    static constexpr size_t kBytes = size_t(32) << 20;
+    float *dev = nullptr;
+    float *host = nullptr;              // pinned
+    size_t head = 0;                    // next free byte
+    struct Rec {
+        size_t off, bytes;
+        hipEvent_t ev;
+    };
+    std::deque<Rec> live;
+    std::vector<hipEvent_t> spare;
+};
+
+}  // namespace
+
+void free_operand_ring(qsv_state *s)
+{
+    OperandRing *r = reinterpret_cast<OperandRing *>(s->op_ring);
+    if (!r) return;
+    for (auto &x : r->live) (void)hipEventDestroy(x.ev);
+    for (hipEvent_t e : r->spare) (void)hipEventDestroy(e);
+    if (r->dev) (void)hipFree(r->dev);
+    if (r->host) (void)hipHostFree(r->host);
+    delete r;
+    s->op_ring = nullptr;
+}
+
+namespace {
+
+// Room for `bytes` of operands in the rings: returns the byte offset, after
+// waiting for the windows whose operands occupied that range.
+int ring_alloc(qsv_state *s, size_t bytes, size_t *off)
+{
+    OperandRing *r = reinterpret_cast<OperandRing *>(s->op_ring);
+    if (!r) {
+        r = new OperandRing;
+        s->op_ring = r;
+        if (hipMalloc((void **)&r->dev, OperandRing::kBytes) != hipSuccess) {
+            r->dev = nullptr;
+            free_operand_ring(s);
+            return fail(QSV_ERR_ALLOC, "cannot allocate the operand ring");
+        }
+        if (hipHostMalloc((void **)&r->host, OperandRing::kBytes, hipHostMallocDefault) != hipSuccess) {
+            r->host = nullptr;
+            free_operand_ring(s);
+            return fail(QSV_ERR_ALLOC, "cannot allocate the pinned operand ring");
+        }
+    }
+    if (bytes > OperandRing::kBytes) return fail(QSV_ERR_ARG, "window operands exceed the operand ring");
+    bytes = (bytes + 255) & ~size_t(255);
+    if (r->head + bytes > OperandRing::kBytes) r->head = 0;
+    const size_t a = r->head, b = a + bytes;
+    for (auto it = r->live.begin(); it != r->live.end();) {
+        if (it->off < b && a < it->off + it->bytes) {
+            QSV_HIP(hipEventSynchronize(it->ev));
+            r->spare.push_back(it->ev);
+            it = r->live.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    r->head = b;
+    *off = a;
+    return QSV_OK;
+}
+
+int ring_done(qsv_state *s, size_t off, size_t bytes)
+{
+    OperandRing *r = reinterpret_cast<OperandRing *>(s->op_ring);
+    hipEvent_t ev;
+    if (!r->spare.empty()) {
+        ev = r->spare.back();
+        r->spare.pop_back();
+    } else {
+        QSV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    const hipError_t e = hipEventRecord(ev, s->stream);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        return hip_fail(e, "hipEventRecord");
+    }
+    r->live.push_back({off, (bytes + 255) & ~size_t(255), ev});
+    return QSV_OK;
+}
+
+int run_streamed(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats, const qsv_run_opts &o,
+                 bool learned, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms, Profiler *prof,
+                 hipEvent_t start, std::chrono::steady_clock::time_point t_plan, double *plan_ms, int *k_used,
+                 int *planner)
+{
+    WindowQueue q(64);
+    std::thread producer(produce_windows, s, gates, ngates, mats, o, learned, &q);
+    // the FP16x3 state scale is measured while the first windows are planned
+    int rc = o.split == 3 ? f16_state_scale(s) : QSV_OK;
+    bool first = true;
+    int64_t nblocks = 0;
+    double pred = 0;
+    int kmax = 0;
+    if (!rc && prof->on()) {
+        s->tlog_feats.clear();
+        s->tlog_ms.clear();
+    }
+    while (!rc) {
+        ReadyWindow w = q.pop();
+        if (w.rc) {
+            rc = fail(w.rc, w.err);
+            break;
+        }
+        if (w.end) break;
+        if (first) {
+            *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
+            const hipError_t e = hipEventRecord(start, s->stream);    // device time starts with the first window
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "hipEventRecord");
+                break;
+            }
+            first = false;
+        }
+        if (w.barrier >= 0) {
+            rc = exec_launches(s, gates, w.plan, w.mfma, w.launches, nullptr, w.op_off, o, rng, outcomes, prof,
+                               &nblocks);
+            continue;
+        }
+        const size_t bytes = w.ops.size() * sizeof(float);
+        size_t off = 0;
+        const float *dops = nullptr;
+        if (bytes) {
+            if ((rc = ring_alloc(s, bytes, &off))) break;
+            OperandRing *r = reinterpret_cast<OperandRing *>(s->op_ring);
+            std::memcpy(reinterpret_cast<char *>(r->host) + off, w.ops.data(), bytes);
+            const hipError_t e = hipMemcpyAsync(reinterpret_cast<char *>(r->dev) + off,
+                                                reinterpret_cast<char *>(r->host) + off, bytes,
+                                                hipMemcpyHostToDevice, s->stream);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "operand upload");
+                break;
+            }
+            dops = reinterpret_cast<const float *>(reinterpret_cast<const char *>(r->dev) + off);
+        }
+        rc = exec_launches(s, gates, w.plan, w.mfma, w.launches, dops, w.op_off, o, rng, outcomes, prof, &nblocks);
+        if (!rc && bytes) rc = ring_done(s, off, bytes);
+        pred += w.pred;
+        kmax = std::max(kmax, w.k);
+    }
+    if (rc) q.stop();
+    producer.join();
+    if (first) *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
+    *blocks = nblocks;
+    *pred_ms = pred;
+    *k_used = kmax ? kmax : std::max(4, std::min(o.fuse_k, 5));
+    *planner = 1;
+    return rc;
+}
+
+}  // namespace
+
+int run_fused(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats,
+              const qsv_run_opts &o, int32_t *rng, int *outcomes, int64_t *blocks, double *pred_ms,
+              Profiler *prof, hipEvent_t start, double *plan_ms, int *k_used, int *planner)
+{
+    if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "the fused path runs on c64 states");
+    const auto t_plan = std::chrono::steady_clock::now();
+    if (o.tile != 0 && (o.tile < 6 || o.tile > 8)) return fail(QSV_ERR_ARG, "tile column bits must be 6, 7 or 8");
+    if (o.pass_tb != 0 && (o.pass_tb < 11 || o.pass_tb > kMaxPassTileBits))
+        return fail(QSV_ERR_ARG, "pass tile bits must be 0 or 11..13");
+    const bool windowed = o.scheduler == 3 && s->n >= 11;
+    // window schedules of states that hold a 13-bit tile window run as a
+    // planning / sweep pipeline (the learned schedule with its beam search)
+    const bool learned_windows = o.scheduler == 1 && s->cm && s->n >= 11;
+    if (windowed || learned_windows)
+        return run_streamed(s, gates, ngates, mats, o, learned_windows, rng, outcomes, blocks, pred_ms, prof, start,
+                            t_plan, plan_ms, k_used, planner);
+    std::vector<PlanItem> plan;
+    std::vector<int> window;
+    int rc;
+    SchedChoice choice;
+    if (o.scheduler == 1) {
+        rc = plan_learned(s, gates, ngates, mats, o, plan, &choice, &window);
+    } else {
+        rc = plan_greedy(gates, ngates, mats, std::min(o.fuse_k, 5), o.exact, plan);
+        choice.fuse_k = o.fuse_k;
+    }
+    if (rc) return rc;
+    std::vector<char> mfma;
+    prepare_blocks(s, plan, window, mfma);
+    std::vector<Launch> launches;
+    double pred = 0;
+    if (o.scheduler == 1 && s->cm) {
+        rc = group_launches(s, plan, o, launches, &pred, window.empty() ? nullptr : &window);
+    } else {
+        qsv_run_opts g = o;
+        if (o.scheduler != 2) g.group = 1;
+        rc = group_launches(s, plan, g, launches, &pred);
+    }
+    if (rc) return rc;
+    std::vector<size_t> op_off;
+    std::vector<float> ops;
+    build_operands(plan, mfma, o.split, 8, op_off, ops);
+    float *dops = nullptr;
+    if (!ops.empty()) {
+        if (hipMalloc((void **)&dops, ops.size() * sizeof(float)) != hipSuccess)
+            return fail(QSV_ERR_ALLOC, "cannot allocate block operands");
+        hipError_t e = hipMemcpyAsync(dops, ops.data(), ops.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(dops);
+            return hip_fail(e, "operand upload");
+        }
+    }
+    if (o.split == 3 && (rc = f16_state_scale(s))) {
+        if (dops) (void)hipFree(dops);
+        return rc;
+    }
+    int64_t nblocks = 0;
+    *plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan).count();
+    *k_used = choice.fuse_k;
+    *planner = choice.window;
+    QSV_HIP(hipEventRecord(start, s->stream));       // device time starts after planning
+    if (prof->on()) {
+        s->tlog_feats.clear();
+        s->tlog_ms.clear();
+    }
+    rc = exec_launches(s, gates, plan, mfma, launches, dops, op_off, o, rng, outcomes, prof, &nblocks);
     if (dops) {
         (void)hipStreamSynchronize(s->stream);
         (void)hipFree(dops);

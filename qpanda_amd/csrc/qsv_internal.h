@@ -66,6 +66,7 @@ struct qsv_state {
     double *red = nullptr;           // device reduction scratch (kRedSlots doubles)
     double *host_red = nullptr;      // pinned host mirror
     void *aux = nullptr;             // device scratch for readouts (marginal partials)
+    void *op_ring = nullptr;         // operand rings of the streamed window schedules (qsv_fusion.cpp)
     size_t aux_bytes = 0;
     double *mats = nullptr;          // device copy of a circuit's matrix pool
     size_t mats_bytes = 0;
@@ -143,6 +144,9 @@ int gather(qsv_state *s, const uint64_t *idx, int64_t count, double *out);
 int launch_collapse(qsv_state *s, int q, int outcome, double scale);
 int marginals(qsv_state *s, const int *qubits, int m, double *out);
 int pmeasure_top(qsv_state *s, const int *qubits, int m, uint64_t *idx, double *prob, int *count);
+
+// Releases the operand rings of the streamed window schedules (qsv_fusion.cpp).
+void free_operand_ring(qsv_state *s);
 
 // Fused FP32 MFMA block (qsv_fused.hip).
 int launch_block_mfma(qsv_state *s, const int *qubits, int k, const double *u, int split);

@@ -45,6 +45,38 @@ void absorb_blocks(std::vector<PlanItem> &plan, int max_k, bool shape_only = fal
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
                 std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only = false,
                 const std::vector<uint64_t> *masks_in = nullptr, std::vector<uint64_t> *masks_out = nullptr);
+// One window's records fused into blocks of <= max_k qubits: taken = the
+// records (indices into gates) in a dependency order; the closure's order and
+// program order are both fused and the one with fewer blocks is kept.
+int fuse_window(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k, int exact,
+                bool shape_only, std::vector<PlanItem> &blocks);
+
+// The windows of a gate list one at a time (qsv_window.cpp): a window's
+// qubit mask and records, or a MEASURE / RESET record on its own.
+struct WindowUnit {
+    int64_t barrier = -1;              // record index of a MEASURE / RESET, else -1
+    uint64_t mask = 0;                 // the window's qubits
+    std::vector<int64_t> taken;        // its records, in a dependency order
+};
+struct BeamOpts {
+    int width = 1;                     // states kept per depth (1: plan_window's windows)
+    int branch = 3;                    // windows expanded per state
+    int first = 6;                     // first growth candidates completed per state
+    int delay = 4;                     // windows a commitment trails the search
+};
+class WindowStream {
+public:
+    WindowStream(const qsv_gate *gates, int64_t ng, int n, int tb, const BeamOpts &b);
+    ~WindowStream();
+    WindowStream(const WindowStream &) = delete;
+    WindowStream &operator=(const WindowStream &) = delete;
+    int next(WindowUnit &u, bool *end);          // *end: no units left
+    int64_t windows() const;                     // units handed out so far
+private:
+    struct Impl;
+    Impl *impl_;
+};
+
 // Qubits that widen block i of a window plan to kq qubits: the highest of
 // its window's qubits (the union of the window's blocks and qubits 0..3)
 // that it does not use.

@@ -29,7 +29,14 @@
 #include "qsv_sched.h"
 
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
+#include <unordered_set>
 
 namespace qsv {
 
@@ -175,15 +182,14 @@ struct Segment {
         return out;
     }
 
-    // Growth with lookahead: at each step, complete the growth greedily from
-    // each of the `look` best candidates and take the one whose completed
-    // window runs the most records (ties: the better first step).
-    uint64_t grow(int tb, int look) const
+    // Growth with lookahead from window w with queue positions `from`: at each
+    // step, complete the growth greedily from each of the `look` best
+    // candidates and take the one whose completed window runs the most records
+    // (ties: the better first step).
+    uint64_t grow_from(uint64_t w, const size_t *from, int tb, int look) const
     {
-        uint64_t w = 0;
-        for (int q = 0; q < std::min(4, n); q++) w |= uint64_t(1) << q;
         size_t h[64], h2[64];
-        std::copy(head.begin(), head.end(), h);
+        std::copy(from, from + n, h);
         closure(w, h, nullptr);
         std::vector<std::pair<double, uint64_t>> cs;
         std::unordered_map<uint64_t, uint64_t> memo;     // greedy completions
@@ -211,7 +217,7 @@ struct Segment {
                     if (r != runs.end()) {
                         got = r->second;
                     } else {
-                        std::copy(head.begin(), head.end(), h2);
+                        std::copy(from, from + n, h2);
                         got = runs[full] = closure(full, h2, nullptr);
                     }
                     if (got > best) {
@@ -225,9 +231,164 @@ struct Segment {
         }
         return w;
     }
+
+    uint64_t base_window() const
+    {
+        uint64_t w = 0;
+        for (int q = 0; q < std::min(4, n); q++) w |= uint64_t(1) << q;
+        return w;
+    }
+
+    uint64_t grow(int tb, int look) const { return grow_from(base_window(), head.data(), tb, look); }
+
+    // The starting windows of a beam state's expansion from positions
+    // `from`: the base window (whose lookahead growth is the heuristic's own
+    // window) and the base plus each of its `first` best first candidates.
+    void starts(const size_t *from, int tb, int first, std::vector<uint64_t> &out) const
+    {
+        const uint64_t w0 = base_window();
+        out.assign(1, w0);
+        size_t h[64];
+        std::copy(from, from + n, h);
+        closure(w0, h, nullptr);
+        std::vector<std::pair<double, uint64_t>> cs;
+        candidates(w0, h, tb - __builtin_popcountll(w0), cs);
+        std::stable_sort(cs.begin(), cs.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        for (int c = 0; c < (int)cs.size() && c < first && cs[c].first > 0; c++) out.push_back(w0 | cs[c].second);
+    }
+};
+
+// The records [i, j) (no MEASURE / RESET among them) as a segment: per qubit,
+// the records that act on it, in order.
+int build_segment(const qsv_gate *gates, int64_t i, int64_t j, int n, Segment &sg)
+{
+    sg.gates = gates + i;
+    sg.count = j - i;
+    sg.n = n;
+    sg.nq.resize(sg.count);
+    sg.qs.assign(3 * sg.count, 0);
+    sg.queue.assign(n, {});
+    sg.head.assign(n, 0);
+    for (int64_t g = 0; g < sg.count; g++) {
+        const int k = record_qubits(sg.gates[g], &sg.qs[3 * g]);
+        if (k < 0) return fail(QSV_ERR_UNSUPPORTED, "gate kind cannot be fused");
+        sg.nq[g] = k;
+        for (int x = 0; x < k; x++) {
+            const int q = sg.qs[3 * g + x];
+            if (q < 0 || q >= n) return fail(QSV_ERR_ARG, "qubit out of range");
+            sg.queue[q].push_back(g);
+        }
+    }
+    return QSV_OK;
+}
+
+bool is_barrier(const qsv_gate &r) { return r.kind == QSV_GATE_MEASURE || r.kind == QSV_GATE_RESET; }
+
+// A few worker threads for the beam search: run(n, f) calls f(0) .. f(n - 1)
+// on the workers and the calling thread and returns when all have returned.
+class JobPool {
+public:
+    explicit JobPool(int workers)
+    {
+        for (int i = 0; i < workers; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~JobPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : th_) t.join();
+    }
+    void run(size_t n, const std::function<void(size_t)> &f)
+    {
+        if (th_.empty() || n <= 1) {
+            for (size_t i = 0; i < n; i++) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            f_ = &f;
+            n_ = n;
+            next_ = 0;
+            done_ = 0;
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return done_ == n_; });
+        f_ = nullptr;
+    }
+
+private:
+    void work()
+    {
+        for (;;) {
+            size_t i;
+            const std::function<void(size_t)> *f;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!f_ || next_ >= n_) return;
+                i = next_++;
+                f = f_;
+            }
+            (*f)(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (++done_ == n_) done_cv_.notify_all();
+        }
+    }
+    void loop()
+    {
+        int seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return quit_ || gen_ != seen; });
+                if (quit_) return;
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)> *f_ = nullptr;
+    size_t n_ = 0, next_ = 0, done_ = 0;
+    int gen_ = 0;
+    bool quit_ = false;
 };
 
 }  // namespace
+
+int fuse_window(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k, int exact,
+                bool shape_only, std::vector<PlanItem> &blocks)
+{
+    // greedy fusion depends on the order it sees the gates in: fuse the
+    // window's gates in the closure's order and in program order (as shapes)
+    // and keep the order that gives fewer blocks
+    std::vector<qsv_gate> sub, sub_prog;
+    std::vector<PlanItem> blocks_prog;
+    for (int64_t g : taken) sub.push_back(gates[g]);
+    std::vector<int64_t> sorted(taken);
+    std::sort(sorted.begin(), sorted.end());
+    for (int64_t g : sorted) sub_prog.push_back(gates[g]);
+    int rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, true);
+    if (!rc) rc = plan_greedy(sub_prog.data(), (int64_t)sub_prog.size(), mats, max_k, exact, blocks_prog, true);
+    if (rc) return rc;
+    if (blocks_prog.size() < blocks.size()) sub.swap(sub_prog);
+    if (!shape_only || blocks_prog.size() < blocks.size())
+        rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, shape_only);
+    if (rc) return rc;
+    // every record is counted by some block (no-op records as the identity on
+    // their qubit), one draw each
+    int64_t counted = 0;
+    for (const PlanItem &b : blocks) counted += b.ngates;
+    if (counted != (int64_t)sub.size()) return fail(QSV_ERR_ARG, "internal: window records not all counted");
+    return QSV_OK;
+}
 
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
                 std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only,
@@ -254,27 +415,11 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
         int64_t j = i;
         while (j < ng && gates[j].kind != QSV_GATE_MEASURE && gates[j].kind != QSV_GATE_RESET) j++;
         Segment sg;
-        sg.gates = gates + i;
-        sg.count = j - i;
-        sg.n = n;
-        sg.nq.resize(sg.count);
-        sg.qs.assign(3 * sg.count, 0);
-        sg.queue.assign(n, {});
-        sg.head.assign(n, 0);
-        for (int64_t g = 0; g < sg.count; g++) {
-            const int k = record_qubits(sg.gates[g], &sg.qs[3 * g]);
-            if (k < 0) return fail(QSV_ERR_UNSUPPORTED, "gate kind cannot be fused");
-            sg.nq[g] = k;
-            for (int x = 0; x < k; x++) {
-                const int q = sg.qs[3 * g + x];
-                if (q < 0 || q >= n) return fail(QSV_ERR_ARG, "qubit out of range");
-                sg.queue[q].push_back(g);
-            }
-        }
+        int rc0 = build_segment(gates, i, j, n, sg);
+        if (rc0) return rc0;
         int64_t left = sg.count;
         std::vector<int64_t> taken;
-        std::vector<qsv_gate> sub, sub_prog;
-        std::vector<PlanItem> blocks, blocks_prog;
+        std::vector<PlanItem> blocks;
         while (left > 0) {
             size_t hc[64];
             uint64_t w;
@@ -291,26 +436,9 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
             std::copy(hc, hc + n, sg.head.begin());
             if (got == 0) return fail(QSV_ERR_ARG, "window planner made no progress");
             left -= got;
-            // greedy fusion depends on the order it sees the gates in: fuse
-            // the window's gates in the closure's order and in program order
-            // (as shapes) and keep the order that gives fewer blocks
-            sub.clear();
-            for (int64_t g : taken) sub.push_back(sg.gates[g]);
-            std::sort(taken.begin(), taken.end());
-            sub_prog.clear();
-            for (int64_t g : taken) sub_prog.push_back(sg.gates[g]);
-            int rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, true);
-            if (!rc) rc = plan_greedy(sub_prog.data(), (int64_t)sub_prog.size(), mats, max_k, exact, blocks_prog, true);
+            for (int64_t &g : taken) g += i;
+            int rc = fuse_window(gates, taken, mats, max_k, exact, shape_only, blocks);
             if (rc) return rc;
-            if (blocks_prog.size() < blocks.size()) sub.swap(sub_prog);
-            if (!shape_only || blocks_prog.size() < blocks.size())
-                rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, shape_only);
-            if (rc) return rc;
-            // every record is counted by some block (no-op records as the
-            // identity on their qubit), one draw each
-            int64_t counted = 0;
-            for (const PlanItem &b : blocks) counted += b.ngates;
-            if (counted != (int64_t)sub.size()) return fail(QSV_ERR_ARG, "internal: window records not all counted");
             for (PlanItem &b : blocks) {
                 plan.push_back(std::move(b));
                 window.push_back(wid);
@@ -319,6 +447,194 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
         }
         i = j;
     }
+    return QSV_OK;
+}
+
+// Window stream: the windows of the plan in order, one at a time, so that the
+// caller can run a window while later ones are planned.  With width 1 these
+// are plan_window's windows (the lookahead growth from the positions the
+// committed windows left).  With width > 1 a beam search looks for windows
+// that serve later windows better than the most gates now: every state (the
+// queue positions after a sequence of windows) expands into the `branch`
+// windows that run the most records among the lookahead growth's own window
+// and the completions from its `first` best first candidates; the `width`
+// states that have run the most records survive each depth (duplicates of
+// the same positions once).  A window is committed once the best state is
+// `delay` windows past it (fewer for the first windows of a segment, so the
+// first sweep starts early); states that disagree with it are dropped.  The
+// expansions of a depth run on a few worker threads.  On the 30-qubit
+// depth-200 benchmark circuit: 234 windows with width 1, 221 with width 4,
+// branch 3, first 6, delay 4, 216 with width 12, branch 4, first 8, delay 6
+// (another seed: 238, 223, 219).
+struct WindowStream::Impl {
+    const qsv_gate *gates = nullptr;
+    int64_t ng = 0;
+    int n = 0, tb = 0;
+    BeamOpts b;
+    int64_t next_rec = 0;              // first record not yet in a segment
+    bool active = false;
+    int64_t seg0 = 0;                  // the active segment's first record
+    Segment sg;
+    std::vector<size_t> committed;     // queue positions after the committed windows
+    int64_t ncommit = 0;               // windows committed in the active segment
+    int64_t total = 0;                 // windows committed in all
+    struct State {
+        std::vector<size_t> head;
+        int64_t done = 0;
+        std::vector<uint64_t> path;    // windows not yet committed
+    };
+    std::vector<State> beam;
+    std::deque<WindowUnit> out;
+    std::unique_ptr<JobPool> pool;
+
+    void commit(uint64_t w)
+    {
+        WindowUnit u;
+        u.mask = w;
+        std::vector<int64_t> t;
+        sg.closure(w, committed.data(), &t);
+        u.taken.reserve(t.size());
+        for (int64_t g : t) u.taken.push_back(g + seg0);
+        out.push_back(std::move(u));
+        ncommit++;
+        total++;
+    }
+
+    int start_segment()
+    {
+        int64_t j = next_rec;
+        while (j < ng && !is_barrier(gates[j])) j++;
+        int rc = build_segment(gates, next_rec, j, n, sg);
+        if (rc) return rc;
+        seg0 = next_rec;
+        next_rec = j;
+        committed.assign(n, 0);
+        ncommit = 0;
+        beam.assign(1, State());
+        beam[0].head.assign(n, 0);
+        active = true;
+        return QSV_OK;
+    }
+
+    int step()
+    {
+        std::vector<State> next;
+        std::unordered_set<uint64_t> seen;
+        // every state's starting windows, grown with lookahead on the pool
+        struct Job {
+            size_t st;
+            uint64_t w0, w = 0;
+            int64_t got = 0;
+        };
+        std::vector<Job> jobs;
+        std::vector<uint64_t> ws;
+        for (size_t i = 0; i < beam.size(); i++) {
+            if (b.width <= 1) ws.assign(1, sg.base_window());
+            else sg.starts(beam[i].head.data(), tb, b.first, ws);
+            for (uint64_t w0 : ws) jobs.push_back({i, w0});
+        }
+        if (b.width > 1 && !pool) {
+            const int hw = (int)std::thread::hardware_concurrency();
+            pool.reset(new JobPool(std::max(0, std::min(6, hw - 2))));
+        }
+        auto grow = [&](size_t j) {
+            Job &jb = jobs[j];
+            const size_t *from = beam[jb.st].head.data();
+            jb.w = sg.grow_from(jb.w0, from, tb, kLookahead);
+            size_t h[64];
+            std::copy(from, from + n, h);
+            jb.got = sg.closure(jb.w, h, nullptr);
+        };
+        if (pool) pool->run(jobs.size(), grow);
+        else
+            for (size_t j = 0; j < jobs.size(); j++) grow(j);
+        // each state's `branch` best distinct windows
+        for (size_t i = 0, j = 0; i < beam.size(); i++) {
+            std::vector<std::pair<int64_t, uint64_t>> fin;
+            for (; j < jobs.size() && jobs[j].st == i; j++) {
+                bool dup = false;
+                for (const auto &x : fin) dup |= x.second == jobs[j].w;
+                if (!dup) fin.push_back({jobs[j].got, jobs[j].w});
+            }
+            std::stable_sort(fin.begin(), fin.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+            const State &st = beam[i];
+            for (int f = 0; f < (int)fin.size() && f < std::max(1, b.branch); f++) {
+                const uint64_t w = fin[f].second;
+                State c = st;
+                const int64_t got = sg.closure(w, c.head.data(), nullptr);
+                if (got == 0) continue;
+                c.done += got;
+                c.path.push_back(w);
+                uint64_t hsh = 1469598103934665603ull;
+                for (size_t x : c.head) hsh = (hsh ^ x) * 1099511628211ull;
+                if (!seen.insert(hsh).second) continue;
+                next.push_back(std::move(c));
+            }
+        }
+        if (next.empty()) return fail(QSV_ERR_ARG, "window planner made no progress");
+        std::stable_sort(next.begin(), next.end(), [](const State &x, const State &y) { return x.done > y.done; });
+        if (next[0].done == sg.count) {
+            for (uint64_t w : next[0].path) commit(w);
+            beam.clear();
+            active = false;
+            return QSV_OK;
+        }
+        if ((int)next.size() > b.width) next.resize(b.width);
+        const size_t delay = b.width <= 1 ? 0 : (size_t)std::min<int64_t>(b.delay, ncommit + 1);
+        while (next[0].path.size() > delay) {
+            const uint64_t w = next[0].path[0];
+            commit(w);
+            std::vector<State> keep;
+            for (State &st : next)
+                if (st.path[0] == w) {
+                    st.path.erase(st.path.begin());
+                    keep.push_back(std::move(st));
+                }
+            next.swap(keep);
+        }
+        beam.swap(next);
+        return QSV_OK;
+    }
+};
+
+WindowStream::WindowStream(const qsv_gate *gates, int64_t ng, int n, int tb, const BeamOpts &b) : impl_(new Impl)
+{
+    impl_->gates = gates;
+    impl_->ng = ng;
+    impl_->n = n;
+    impl_->tb = std::min(tb, n);
+    impl_->b = b;
+}
+
+WindowStream::~WindowStream() { delete impl_; }
+
+int64_t WindowStream::windows() const { return impl_->total; }
+
+int WindowStream::next(WindowUnit &u, bool *end)
+{
+    Impl &m = *impl_;
+    *end = false;
+    if (m.n < 4 || m.n > 62) return fail(QSV_ERR_ARG, "window planning needs 4..62 qubits");
+    while (m.out.empty()) {
+        if (!m.active) {
+            if (m.next_rec >= m.ng) {
+                *end = true;
+                return QSV_OK;
+            }
+            if (is_barrier(m.gates[m.next_rec])) {
+                u = WindowUnit();
+                u.barrier = m.next_rec++;
+                m.total++;
+                return QSV_OK;
+            }
+            int rc = m.start_segment();
+            if (rc) return rc;
+        }
+        int rc = m.step();
+        if (rc) return rc;
+    }
+    u = std::move(m.out.front());
+    m.out.pop_front();
     return QSV_OK;
 }
 
@@ -349,20 +665,11 @@ std::vector<int> window_widening(const std::vector<PlanItem> &plan, const std::v
 
 using namespace qsv;
 
-extern "C" int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
-                               int tb, int n, int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
-                               int64_t *ngates_out, int *window_out, int64_t *nblocks)
+// Both host planners' output format (k_out .. nblocks as qsv_fuse_plan, plus
+// window_out).
+static int emit_plan(const std::vector<PlanItem> &plan, const std::vector<int> &window, int64_t max_blocks, int *k_out,
+                     int *qubits_out, double *mats_out, int64_t *ngates_out, int *window_out, int64_t *nblocks)
 {
-    if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
-    if (tb < 4 || tb > kMaxPassTileBits) return fail(QSV_ERR_ARG, "window tile bits must be 4..13");
-    // a window holds qubits 0..3 and grows from there, so it needs room for
-    // the widest record (3 qubits: Toffoli, Fredkin) beside them, unless it
-    // holds the whole state
-    if (tb < 7 && tb < n) return fail(QSV_ERR_ARG, "window tile bits must be >= 7 below the state size");
-    std::vector<PlanItem> plan;
-    std::vector<int> window;
-    int rc = plan_window(gates, ngates, mats, max_k, exact, tb, n, plan, window);
-    if (rc) return rc;
     *nblocks = (int64_t)plan.size();
     if ((int64_t)plan.size() > max_blocks) return fail(QSV_ERR_ARG, "plan larger than the output buffers");
     for (size_t b = 0; b < plan.size(); b++) {
@@ -377,4 +684,64 @@ extern "C" int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const doub
             for (size_t j = 0; j < it.u.size(); j++) mats_out[b * 2048 + j] = it.u[j];
     }
     return QSV_OK;
+}
+
+extern "C" int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
+                                    int tb, int n, int width, int branch, int first, int delay, int64_t max_blocks,
+                                    int *k_out, int *qubits_out, double *mats_out, int64_t *ngates_out,
+                                    int *window_out, int64_t *nblocks)
+{
+    if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
+    if (tb < 4 || tb > kMaxPassTileBits) return fail(QSV_ERR_ARG, "window tile bits must be 4..13");
+    if (tb < 7 && tb < n) return fail(QSV_ERR_ARG, "window tile bits must be >= 7 below the state size");
+    if (max_k < 4 || max_k > 5) return fail(QSV_ERR_ARG, "window planning fuses blocks of 4 or 5 qubits");
+    if (width < 1 || width > 64 || branch < 1 || branch > 16 || first < 0 || first > 32 || delay < 0 || delay > 64)
+        return fail(QSV_ERR_ARG, "beam width 1..64, branch 1..16, first 0..32, delay 0..64");
+    BeamOpts bo;
+    bo.width = width;
+    bo.branch = branch;
+    bo.first = first;
+    bo.delay = delay;
+    WindowStream ws(gates, ngates, n, tb, bo);
+    std::vector<PlanItem> plan, blocks;
+    std::vector<int> window;
+    int wid = 0;
+    for (;;) {
+        WindowUnit u;
+        bool end = false;
+        int rc = ws.next(u, &end);
+        if (rc) return rc;
+        if (end) break;
+        if (u.barrier >= 0) {
+            PlanItem it;
+            it.barrier = u.barrier;
+            plan.push_back(std::move(it));
+            window.push_back(wid++);
+            continue;
+        }
+        if ((rc = fuse_window(gates, u.taken, mats, max_k, exact, false, blocks))) return rc;
+        for (PlanItem &b : blocks) {
+            plan.push_back(std::move(b));
+            window.push_back(wid);
+        }
+        wid++;
+    }
+    return emit_plan(plan, window, max_blocks, k_out, qubits_out, mats_out, ngates_out, window_out, nblocks);
+}
+
+extern "C" int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
+                               int tb, int n, int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                               int64_t *ngates_out, int *window_out, int64_t *nblocks)
+{
+    if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
+    if (tb < 4 || tb > kMaxPassTileBits) return fail(QSV_ERR_ARG, "window tile bits must be 4..13");
+    // a window holds qubits 0..3 and grows from there, so it needs room for
+    // the widest record (3 qubits: Toffoli, Fredkin) beside them, unless it
+    // holds the whole state
+    if (tb < 7 && tb < n) return fail(QSV_ERR_ARG, "window tile bits must be >= 7 below the state size");
+    std::vector<PlanItem> plan;
+    std::vector<int> window;
+    int rc = plan_window(gates, ngates, mats, max_k, exact, tb, n, plan, window);
+    if (rc) return rc;
+    return emit_plan(plan, window, max_blocks, k_out, qubits_out, mats_out, ngates_out, window_out, nblocks);
 }

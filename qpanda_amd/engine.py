@@ -29,6 +29,7 @@ EXPORTED = [
     "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
     "qsv_sample_shots", "qsv_sample_shots_copy", "qsv_gather", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
     "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan", "qsv_window_plan",
+    "qsv_window_plan_beam",
     "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
     "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
     "qsv_timing_log", "qsv_batch_create", "qsv_batch_destroy", "qsv_batch_autotune", "qsv_batch_run",
@@ -137,6 +138,8 @@ def lib():
                                C.POINTER(C.c_int64)], I),
             "qsv_window_plan": ([vp, C.c_int64, vp, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
                                  C.POINTER(C.c_int64)], I),
+            "qsv_window_plan_beam": ([vp, C.c_int64, vp, I, I, I, I, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
+                                      C.POINTER(C.c_int64)], I),
             "qsv_cm_create": ([U64, C.POINTER(vp)], I),
             "qsv_cm_destroy": ([vp], I),
             "qsv_cm_nparams": ([], I),
@@ -558,10 +561,11 @@ def fuse_plan(circ: "_circ.Circuit", max_k: int, exact: bool = False):
     return out
 
 
-def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = False):
+def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = False, beam=None):
     """Window plan (host code; no device needed): list of (qubits, matrix,
     ngates, window) blocks, or (None, None, record_index, window) for
-    MEASURE/RESET records, in emission order."""
+    MEASURE/RESET records, in emission order.  beam = (width, branch, first,
+    delay) plans with the beam search (qsv_window_plan_beam)."""
     g = np.ascontiguousarray(circ.gates)
     m = np.ascontiguousarray(circ.mats, np.complex128)
     cap = max(1, g.shape[0])
@@ -571,11 +575,16 @@ def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = F
     ngs = np.zeros(cap, np.int64)
     win = np.zeros(cap, np.int32)
     nb = C.c_int64()
-    _check(lib().qsv_window_plan(g.ctypes.data_as(C.c_void_p), g.shape[0],
-                                 m.ctypes.data_as(C.c_void_p) if m.size else None, max_k, int(exact), tb,
-                                 circ.n, cap, ks.ctypes.data_as(C.c_void_p), qs.ctypes.data_as(C.c_void_p),
-                                 mats.ctypes.data_as(C.c_void_p), ngs.ctypes.data_as(C.c_void_p),
-                                 win.ctypes.data_as(C.c_void_p), C.byref(nb)))
+    outs = (cap, ks.ctypes.data_as(C.c_void_p), qs.ctypes.data_as(C.c_void_p), mats.ctypes.data_as(C.c_void_p),
+            ngs.ctypes.data_as(C.c_void_p), win.ctypes.data_as(C.c_void_p), C.byref(nb))
+    mp = m.ctypes.data_as(C.c_void_p) if m.size else None
+    if beam is None:
+        _check(lib().qsv_window_plan(g.ctypes.data_as(C.c_void_p), g.shape[0], mp, max_k, int(exact), tb,
+                                     circ.n, *outs))
+    else:
+        width, branch, first, delay = (int(v) for v in beam)
+        _check(lib().qsv_window_plan_beam(g.ctypes.data_as(C.c_void_p), g.shape[0], mp, max_k, int(exact), tb,
+                                          circ.n, width, branch, first, delay, *outs))
     out = []
     for b in range(nb.value):
         k = int(ks[b])

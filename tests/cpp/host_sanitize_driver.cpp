@@ -1,7 +1,7 @@
 // host_sanitize_driver.cpp -- drives the index-heavy host code under
 // AddressSanitizer / UndefinedBehaviorSanitizer (built by
 // tests/test_host_sanitizers.py): the greedy block planner (qsv_fuse_plan)
-// and the window planner (qsv_window_plan) on random gate lists of every
+// and the window planners (qsv_window_plan, qsv_window_plan_beam) on random gate lists of every
 // record kind, and the QRunes front end on the reference's example programs,
 // generated programs and byte-level mutations of them.  Any sanitizer report
 // aborts the run; a planner error on a valid list is a failure too.
@@ -131,6 +131,15 @@ void planners(Rng &r, int rounds)
                 int64_t total = 0;
                 for (int64_t b = 0; b < nb && rc == QSV_OK; b++) total += kq[b] ? ng[b] : 1;
                 check(rc != QSV_OK || total == count, "qsv_window_plan gate count");
+                // the beam search over windows (the learned schedule's planner)
+                const int width = 1 + r.below(4), branch = 1 + r.below(3), first = r.below(7), delay = r.below(5);
+                const int rb = qsv_window_plan_beam(g.data(), count, mats.empty() ? nullptr : mats.data(), k, 0, tb,
+                                                    n, width, branch, first, delay, cap, kq.data(), qs.data(),
+                                                    m.data(), ng.data(), win.data(), &nb);
+                check(rb == QSV_OK, "qsv_window_plan_beam");
+                total = 0;
+                for (int64_t b = 0; b < nb && rb == QSV_OK; b++) total += kq[b] ? ng[b] : 1;
+                check(rb != QSV_OK || total == count, "qsv_window_plan_beam gate count");
             }
     }
 }

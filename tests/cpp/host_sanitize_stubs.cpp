@@ -68,4 +68,10 @@ hipError_t hipFree(void *) { return hipErrorNotSupported; }
 hipError_t hipMalloc(void **, size_t) { return hipErrorNotSupported; }
 hipError_t hipMemcpyAsync(void *, const void *, size_t, hipMemcpyKind, hipStream_t) { return hipErrorNotSupported; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipErrorNotSupported; }
+hipError_t hipEventCreateWithFlags(hipEvent_t *, unsigned) { return hipErrorNotSupported; }
+hipError_t hipEventDestroy(hipEvent_t) { return hipErrorNotSupported; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipErrorNotSupported; }
+hipError_t hipHostMalloc(void **, size_t, unsigned) { return hipErrorNotSupported; }
+hipError_t hipHostFree(void *) { return hipErrorNotSupported; }
+hipError_t hipSetDevice(int) { return hipErrorNotSupported; }
 }

@@ -219,3 +219,78 @@ def test_window_plan_tile_bits_range():
         assert sum(ng for _, _, ng, _ in plan) == circ.ngates
     small = C.random_layers(5, 4, 3)
     assert sum(ng for _, _, ng, _ in E.window_plan(small, 4, 5)) == small.ngates
+
+
+# ---- beam search over windows (WindowStream, qsv_window.cpp) -------------------
+
+BEAM = (4, 3, 6, 4)         # the learned schedule's width, branch, first, delay at 29 qubits
+
+
+def test_beam_width_one_is_the_window_plan():
+    """With one state the beam keeps the lookahead growth's windows: the same
+    blocks, matrices and windows as qsv_window_plan."""
+    circ = C.random_layers(16, 30, 4040, haar_frac=0.1)
+    a = E.window_plan(circ, 4, 13)
+    b = E.window_plan(circ, 4, 13, beam=(1, 3, 6, 4))
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x[0] == y[0] and x[2] == y[2] and x[3] == y[3]
+        assert np.array_equal(x[1], y[1])
+
+
+@pytest.mark.parametrize("k", [4, 5])
+@pytest.mark.parametrize("tb", [7, 9, 13])
+def test_beam_plan_reproduces_random_circuit(k, tb):
+    n = 12
+    circ = C.random_layers(n, 12, 700 + k + tb, haar_frac=0.2)
+    psi = C.random_gaussian_state(n, 3 * k + tb)
+    ref = O.State(n)
+    ref.set(psi)
+    ref.run(circ)
+    plan = E.window_plan(circ, k, tb, beam=BEAM)
+    assert sum(ng for _, _, ng, _ in plan) == circ.ngates
+    assert all(len(qs) <= k for qs, _, _, _ in plan)
+    wins = {}
+    for qs, _, _, w in plan:
+        wins.setdefault(w, set(range(4))).update(qs)
+    assert all(len(s) <= max(tb, 4) for s in wins.values())
+    got = apply_window_plan(plan, n, psi)
+    assert np.abs(got.psi - ref.psi).max() < 1e-12
+
+
+def test_beam_plan_every_gate_kind_and_measurements():
+    n = 10
+    b = C._Builder(n)
+    for seed, rec in ((61, (C.MEASURE, 3, 0)), (62, (C.RESET, 8)), (63, None)):
+        part = C.all_gate_kinds(n, seed, reps=2, u1=True)
+        for g in part.gates:
+            k, mi = int(g["kind"]), int(g["mat"])
+            mat = None
+            if mi >= 0:
+                size = 4 if k == C.U1 else 16
+                mat = part.mats[mi: mi + size]
+            b.add(k, int(g["q0"]), int(g["q1"]), int(g["q2"]), theta=float(g["theta"]), dagger=int(g["dagger"]),
+                  mat=mat)
+        if rec:
+            b.add(*rec)
+    circ = b.build()
+    plan = E.window_plan(circ, 4, 8, beam=BEAM)
+    assert sum(p[2] for p in plan if p[0] is not None) == circ.ngates - 2
+    # the barriers in order, each in a window of its own, after exactly the
+    # records that precede them
+    bars = [i for i, p in enumerate(plan) if p[0] is None]
+    assert [plan[i][2] for i in bars] == [int(np.nonzero(circ.gates["kind"] == C.MEASURE)[0][0]),
+                                          int(np.nonzero(circ.gates["kind"] == C.RESET)[0][0])]
+    assert sum(p[2] for p in plan[:bars[0]]) == plan[bars[0]][2]
+    assert all(sum(1 for p in plan if p[3] == plan[i][3]) == 1 for i in bars)
+
+
+def test_beam_needs_fewer_windows_on_the_benchmark_circuit():
+    """30 qubits, depth 200: 234 windows with the lookahead growth alone, at
+    most 225 with the 29-qubit beam (221 measured) and at most 220 with the
+    beam the learned schedule uses from 30 qubits (216 measured)."""
+    circ = C.random_layers(30, 200, 2025, haar_frac=0.1)
+    w1 = len({w for *_, w in E.window_plan(circ, 4, 13)})
+    wb = len({w for *_, w in E.window_plan(circ, 4, 13, beam=BEAM)})
+    w12 = len({w for *_, w in E.window_plan(circ, 4, 13, beam=(12, 4, 8, 6))})
+    assert w12 <= 220 and wb <= 225 < w1, (w12, wb, w1)
