@@ -46,32 +46,36 @@ constexpr int F = QSV_CM_FEATURES, H = 32;
 constexpr int OW1 = 0, OB1 = OW1 + H * F, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
 constexpr int kParams = OB3 + 1;
 
-__global__ void k_mlp_forward(const float *__restrict__ p, const float *__restrict__ x, int64_t batch,
-                              float *__restrict__ y, float *__restrict__ act)
+// Layer 2's sums are accumulated as layer 1's units come out (the same terms
+// in the same order as a row-by-row product, so the same values), which keeps
+// the kernel within 64 VGPRs without spills: 8 waves per SIMD, so that an
+// inference's workgroups fit beside two resident 64 KiB pass workgroups (4
+// waves per SIMD at <= 112 VGPRs) and the scheduler's predictions do not wait
+// for the sweeps they plan.
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(8)))
+void k_mlp_forward(const float *__restrict__ p, const float *__restrict__ x, int64_t batch,
+                   float *__restrict__ y, float *__restrict__ act)
 {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
-    float h1[H], h2[H];
     const float *xb = x + b * F;
-    for (int j = 0; j < H; j++) {
-        float s = p[OB1 + j];
-        for (int i = 0; i < F; i++) s += p[OW1 + j * F + i] * xb[i];
-        h1[j] = s > 0 ? s : 0;
+    float xv[F], s2[H];
+    for (int i = 0; i < F; i++) xv[i] = xb[i];
+    for (int j = 0; j < H; j++) s2[j] = p[OB2 + j];
+    for (int i = 0; i < H; i++) {
+        float s = p[OB1 + i];
+        for (int f = 0; f < F; f++) s += p[OW1 + i * F + f] * xv[f];
+        const float h1 = s > 0 ? s : 0;
+        if (act) act[b * 2 * H + i] = h1;
+        for (int j = 0; j < H; j++) s2[j] += p[OW2 + j * H + i] * h1;
     }
     float out = p[OB3];
     for (int j = 0; j < H; j++) {
-        float s = p[OB2 + j];
-        for (int i = 0; i < H; i++) s += p[OW2 + j * H + i] * h1[i];
-        h2[j] = s > 0 ? s : 0;
-        out += p[OW3 + j] * h2[j];
+        const float h2 = s2[j] > 0 ? s2[j] : 0;
+        out += p[OW3 + j] * h2;
+        if (act) act[b * 2 * H + H + j] = h2;
     }
     y[b] = out;
-    if (act) {
-        for (int j = 0; j < H; j++) {
-            act[b * 2 * H + j] = h1[j];
-            act[b * 2 * H + H + j] = h2[j];
-        }
-    }
 }
 
 __global__ void k_mlp_backward(const float *__restrict__ p, const float *__restrict__ x,

@@ -731,24 +731,6 @@ private:
     bool stop_ = false;
 };
 
-// One window's blocks at block size k (shapes or with matrices), widened and
-// with their launches: the cost model's DP for the learned schedule, one
-// sweep (split at the pass kernel's limit) for the heuristic.
-int plan_one_window(qsv_state *s, const qsv_gate *gates, const double *mats, const qsv_run_opts &o,
-                    const WindowUnit &u, int k, bool shapes, bool learned, std::vector<PlanItem> &plan,
-                    std::vector<char> &mfma, std::vector<Launch> &launches, double *pred)
-{
-    int rc = fuse_window(gates, u.taken, mats, k, o.exact, shapes, plan);
-    if (rc) return rc;
-    const std::vector<int> window(plan.size(), 0);
-    prepare_blocks(s, plan, window, mfma);
-    launches.clear();
-    *pred = 0;
-    if (learned) return group_launches(s, plan, o, launches, pred, &window);
-    window_launches(s, plan, window, mfma, o, launches);
-    return QSV_OK;
-}
-
 void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const double *mats, const qsv_run_opts &o,
                      bool learned, WindowQueue *q)
 {
@@ -781,7 +763,7 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
     const int fixed_k = std::max(4, std::min(o.fuse_k, 5));
     WindowStream ws(gates, ngates, s->n, o.pass_tb ? o.pass_tb : kMaxPassTileBits, bo);
     std::vector<PlanItem> shape, shape5;
-    std::vector<char> shape_mfma, mfma5;
+    std::vector<char> shape_mfma;
     std::vector<Launch> shape_launches, launches5;
     while (!q->stopped()) {
         WindowUnit u;
@@ -808,23 +790,38 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
             continue;
         }
         // the learned schedule predicts the window at k = 4 and k = 5 (as
-        // shapes) and takes k = 5 only for a predicted gain above the model's
-        // resolution; then the blocks get their matrices
+        // shapes, both in one batched inference) and takes k = 5 only for a
+        // predicted gain above the model's resolution; then the blocks get
+        // their matrices
         int k = fixed_k;
         double best = 0;
         if (learned) {
-            double p4 = 0, p5 = 0;
-            if ((rc = plan_one_window(s, gates, mats, o, u, 5, true, true, shape5, mfma5, launches5, &p5)) ||
-                (rc = plan_one_window(s, gates, mats, o, u, 4, true, true, shape, shape_mfma, shape_launches, &p4)))
+            if ((rc = fuse_window(gates, u.taken, mats, 4, o.exact, true, shape)) ||
+                (rc = fuse_window(gates, u.taken, mats, 5, o.exact, true, shape5)))
                 return send_error(rc);
+            const size_t n4 = shape.size();
+            std::vector<int> win(n4, 0);
+            win.resize(n4 + shape5.size(), 1);
+            shape.insert(shape.end(), std::make_move_iterator(shape5.begin()), std::make_move_iterator(shape5.end()));
+            prepare_blocks(s, shape, win, shape_mfma);
+            double total = 0;
+            if ((rc = group_launches(s, shape, o, launches5, &total, &win))) return send_error(rc);
+            double p4 = 0, p5 = 0;
+            shape_launches.clear();
+            for (const Launch &L : launches5) (L.first < n4 ? p4 : p5) += L.pred_ms;
             k = 4;
             best = p4;
             if (p5 < p4 * (1 - kK5Margin)) {
                 k = 5;
                 best = p5;
-                shape.swap(shape5);
-                shape_launches.swap(launches5);
             }
+            for (Launch L : launches5) {
+                if ((L.first < n4) != (k == 4)) continue;
+                if (k == 5) L.first -= n4;
+                shape_launches.push_back(L);
+            }
+            if (k == 5) shape.erase(shape.begin(), shape.begin() + (std::ptrdiff_t)n4);
+            else shape.resize(n4);
         }
         rc = fuse_window(gates, u.taken, mats, k, o.exact, false, w.plan);
         if (rc) return send_error(rc);
