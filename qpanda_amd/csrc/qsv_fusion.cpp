@@ -676,6 +676,8 @@ int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanIte
 // k = 5 only on a predicted gain above the model's per-launch error (at 5%
 // the model picked k = 5 for windows that ran slower)
 constexpr double kK5Margin = 0.10;
+// random dependency orders the learned schedule's fusion tries per window
+constexpr int kFuseTries = 24;
 
 struct ReadyWindow {
     int64_t barrier = -1;
@@ -765,6 +767,7 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
     std::vector<PlanItem> shape, shape5;
     std::vector<char> shape_mfma;
     std::vector<Launch> shape_launches, launches5;
+    std::vector<int64_t> order4, order5;
     while (!q->stopped()) {
         WindowUnit u;
         bool end = false;
@@ -796,8 +799,11 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
         int k = fixed_k;
         double best = 0;
         if (learned) {
-            if ((rc = fuse_window(gates, u.taken, mats, 4, o.exact, true, shape)) ||
-                (rc = fuse_window(gates, u.taken, mats, 5, o.exact, true, shape5)))
+            // fusion searches random dependency orders too (seeded by the
+            // window's first record, so plans repeat run to run)
+            const uint64_t seed = (uint64_t)u.taken[0];
+            if ((rc = best_fusion_order(gates, u.taken, mats, 4, o.exact, kFuseTries, seed, order4, shape)) ||
+                (rc = best_fusion_order(gates, u.taken, mats, 5, o.exact, kFuseTries, seed, order5, shape5)))
                 return send_error(rc);
             const size_t n4 = shape.size();
             std::vector<int> win(n4, 0);
@@ -823,7 +829,8 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
             if (k == 5) shape.erase(shape.begin(), shape.begin() + (std::ptrdiff_t)n4);
             else shape.resize(n4);
         }
-        rc = fuse_window(gates, u.taken, mats, k, o.exact, false, w.plan);
+        rc = learned ? fuse_order(gates, k == 5 ? order5 : order4, mats, k, o.exact, false, w.plan)
+                     : fuse_window(gates, u.taken, mats, k, o.exact, false, w.plan);
         if (rc) return send_error(rc);
         const std::vector<int> window(w.plan.size(), 0);
         prepare_blocks(s, w.plan, window, w.mfma);

@@ -46,10 +46,17 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
                 std::vector<PlanItem> &plan, std::vector<int> &window, bool shape_only = false,
                 const std::vector<uint64_t> *masks_in = nullptr, std::vector<uint64_t> *masks_out = nullptr);
 // One window's records fused into blocks of <= max_k qubits: taken = the
-// records (indices into gates) in a dependency order; the closure's order and
-// program order are both fused and the one with fewer blocks is kept.
+// records (indices into gates) in a dependency order; the closure's order,
+// program order and `tries` random dependency orders (seeded by `seed`) are
+// fused and the first order with the fewest blocks is kept.
 int fuse_window(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k, int exact,
-                bool shape_only, std::vector<PlanItem> &blocks);
+                bool shape_only, std::vector<PlanItem> &blocks, int tries = 0, uint64_t seed = 0);
+// The order fuse_window keeps, with its blocks as shapes.
+int best_fusion_order(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k,
+                      int exact, int tries, uint64_t seed, std::vector<int64_t> &order, std::vector<PlanItem> &shapes);
+// Greedy fusion of records in exactly this order.
+int fuse_order(const qsv_gate *gates, const std::vector<int64_t> &order, const double *mats, int max_k, int exact,
+               bool shape_only, std::vector<PlanItem> &blocks);
 
 // The windows of a gate list one at a time (qsv_window.cpp): a window's
 // qubit mask and records, or a MEASURE / RESET record on its own.

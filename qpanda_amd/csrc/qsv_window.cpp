@@ -29,6 +29,7 @@
 #include "qsv_sched.h"
 
 #include <algorithm>
+#include <array>
 #include <condition_variable>
 #include <deque>
 #include <functional>
@@ -363,24 +364,13 @@ private:
 
 }  // namespace
 
-int fuse_window(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k, int exact,
-                bool shape_only, std::vector<PlanItem> &blocks)
+int fuse_order(const qsv_gate *gates, const std::vector<int64_t> &order, const double *mats, int max_k, int exact,
+               bool shape_only, std::vector<PlanItem> &blocks)
 {
-    // greedy fusion depends on the order it sees the gates in: fuse the
-    // window's gates in the closure's order and in program order (as shapes)
-    // and keep the order that gives fewer blocks
-    std::vector<qsv_gate> sub, sub_prog;
-    std::vector<PlanItem> blocks_prog;
-    for (int64_t g : taken) sub.push_back(gates[g]);
-    std::vector<int64_t> sorted(taken);
-    std::sort(sorted.begin(), sorted.end());
-    for (int64_t g : sorted) sub_prog.push_back(gates[g]);
-    int rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, true);
-    if (!rc) rc = plan_greedy(sub_prog.data(), (int64_t)sub_prog.size(), mats, max_k, exact, blocks_prog, true);
-    if (rc) return rc;
-    if (blocks_prog.size() < blocks.size()) sub.swap(sub_prog);
-    if (!shape_only || blocks_prog.size() < blocks.size())
-        rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, shape_only);
+    std::vector<qsv_gate> sub;
+    sub.reserve(order.size());
+    for (int64_t g : order) sub.push_back(gates[g]);
+    int rc = plan_greedy(sub.data(), (int64_t)sub.size(), mats, max_k, exact, blocks, shape_only);
     if (rc) return rc;
     // every record is counted by some block (no-op records as the identity on
     // their qubit), one draw each
@@ -388,6 +378,103 @@ int fuse_window(const qsv_gate *gates, const std::vector<int64_t> &taken, const 
     for (const PlanItem &b : blocks) counted += b.ngates;
     if (counted != (int64_t)sub.size()) return fail(QSV_ERR_ARG, "internal: window records not all counted");
     return QSV_OK;
+}
+
+int best_fusion_order(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k,
+                      int exact, int tries, uint64_t seed, std::vector<int64_t> &order, std::vector<PlanItem> &shapes)
+{
+    // greedy fusion depends on the order it sees the gates in: fuse the
+    // window's gates in the closure's order, in program order and in `tries`
+    // random dependency orders (as shapes) and keep the first order that gives
+    // the fewest blocks
+    order = taken;
+    int rc = fuse_order(gates, order, mats, max_k, exact, true, shapes);
+    if (rc) return rc;
+    std::vector<int64_t> cand(taken);
+    std::sort(cand.begin(), cand.end());
+    std::vector<PlanItem> other;
+    auto consider = [&]() {
+        int r = fuse_order(gates, cand, mats, max_k, exact, true, other);
+        if (r) return r;
+        if (other.size() < shapes.size()) {
+            shapes.swap(other);
+            order = cand;
+        }
+        return QSV_OK;
+    };
+    if ((rc = consider())) return rc;
+    if (tries <= 0) return QSV_OK;
+    // random orders: each step takes a uniformly chosen record among those at
+    // the front of every queue of their qubits (per-qubit queues in program
+    // order), so every order respects the records' dependencies
+    const std::vector<int64_t> prog(cand);
+    const size_t m = prog.size();
+    std::vector<std::vector<int>> queue(64);
+    std::vector<std::array<int, 3>> rq(m);
+    std::vector<int> nq(m);
+    for (size_t i = 0; i < m; i++) {
+        nq[i] = record_qubits(gates[prog[i]], rq[i].data());
+        if (nq[i] < 0) return fail(QSV_ERR_UNSUPPORTED, "gate kind cannot be fused");
+        for (int j = 0; j < nq[i]; j++) queue[rq[i][j]].push_back((int)i);
+    }
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+    auto next = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    std::vector<size_t> head(64);
+    std::vector<int> ready;
+    std::vector<char> in_ready(m);
+    for (int t = 0; t < tries; t++) {
+        std::fill(head.begin(), head.end(), 0);
+        std::fill(in_ready.begin(), in_ready.end(), 0);
+        ready.clear();
+        auto front = [&](int i) {
+            for (int j = 0; j < nq[i]; j++) {
+                const std::vector<int> &qq = queue[rq[i][j]];
+                if (head[rq[i][j]] >= qq.size() || qq[head[rq[i][j]]] != i) return false;
+            }
+            return true;
+        };
+        for (size_t i = 0; i < m; i++)
+            if (front((int)i)) {
+                ready.push_back((int)i);
+                in_ready[i] = 1;
+            }
+        cand.clear();
+        while (!ready.empty()) {
+            const size_t pick = next() % ready.size();
+            const int i = ready[pick];
+            ready[pick] = ready.back();
+            ready.pop_back();
+            cand.push_back(prog[i]);
+            for (int j = 0; j < nq[i]; j++) {
+                const int q = rq[i][j];
+                head[q]++;
+                if (head[q] < queue[q].size()) {
+                    const int f = queue[q][head[q]];
+                    if (!in_ready[f] && front(f)) {
+                        ready.push_back(f);
+                        in_ready[f] = 1;
+                    }
+                }
+            }
+        }
+        if (cand.size() != m) return fail(QSV_ERR_ARG, "internal: dependency order left records behind");
+        if ((rc = consider())) return rc;
+    }
+    return QSV_OK;
+}
+
+int fuse_window(const qsv_gate *gates, const std::vector<int64_t> &taken, const double *mats, int max_k, int exact,
+                bool shape_only, std::vector<PlanItem> &blocks, int tries, uint64_t seed)
+{
+    std::vector<int64_t> order;
+    int rc = best_fusion_order(gates, taken, mats, max_k, exact, tries, seed, order, blocks);
+    if (rc || shape_only) return rc;
+    return fuse_order(gates, order, mats, max_k, exact, false, blocks);
 }
 
 int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k, int exact, int tb, int n,
@@ -687,16 +774,17 @@ static int emit_plan(const std::vector<PlanItem> &plan, const std::vector<int> &
 }
 
 extern "C" int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
-                                    int tb, int n, int width, int branch, int first, int delay, int64_t max_blocks,
-                                    int *k_out, int *qubits_out, double *mats_out, int64_t *ngates_out,
-                                    int *window_out, int64_t *nblocks)
+                                    int tb, int n, int width, int branch, int first, int delay, int fuse_tries,
+                                    int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                                    int64_t *ngates_out, int *window_out, int64_t *nblocks)
 {
     if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
     if (tb < 4 || tb > kMaxPassTileBits) return fail(QSV_ERR_ARG, "window tile bits must be 4..13");
     if (tb < 7 && tb < n) return fail(QSV_ERR_ARG, "window tile bits must be >= 7 below the state size");
     if (max_k < 4 || max_k > 5) return fail(QSV_ERR_ARG, "window planning fuses blocks of 4 or 5 qubits");
-    if (width < 1 || width > 64 || branch < 1 || branch > 16 || first < 0 || first > 32 || delay < 0 || delay > 64)
-        return fail(QSV_ERR_ARG, "beam width 1..64, branch 1..16, first 0..32, delay 0..64");
+    if (width < 1 || width > 64 || branch < 1 || branch > 16 || first < 0 || first > 32 || delay < 0 || delay > 64 ||
+        fuse_tries < 0 || fuse_tries > 1024)
+        return fail(QSV_ERR_ARG, "beam width 1..64, branch 1..16, first 0..32, delay 0..64, fusion tries 0..1024");
     BeamOpts bo;
     bo.width = width;
     bo.branch = branch;
@@ -719,7 +807,8 @@ extern "C" int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const
             window.push_back(wid++);
             continue;
         }
-        if ((rc = fuse_window(gates, u.taken, mats, max_k, exact, false, blocks))) return rc;
+        if ((rc = fuse_window(gates, u.taken, mats, max_k, exact, false, blocks, fuse_tries, (uint64_t)u.taken[0])))
+            return rc;
         for (PlanItem &b : blocks) {
             plan.push_back(std::move(b));
             window.push_back(wid);

@@ -138,7 +138,7 @@ def lib():
                                C.POINTER(C.c_int64)], I),
             "qsv_window_plan": ([vp, C.c_int64, vp, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
                                  C.POINTER(C.c_int64)], I),
-            "qsv_window_plan_beam": ([vp, C.c_int64, vp, I, I, I, I, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
+            "qsv_window_plan_beam": ([vp, C.c_int64, vp, I, I, I, I, I, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
                                       C.POINTER(C.c_int64)], I),
             "qsv_cm_create": ([U64, C.POINTER(vp)], I),
             "qsv_cm_destroy": ([vp], I),
@@ -565,7 +565,7 @@ def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = F
     """Window plan (host code; no device needed): list of (qubits, matrix,
     ngates, window) blocks, or (None, None, record_index, window) for
     MEASURE/RESET records, in emission order.  beam = (width, branch, first,
-    delay) plans with the beam search (qsv_window_plan_beam)."""
+    delay[, fuse_tries]) plans with the beam search (qsv_window_plan_beam)."""
     g = np.ascontiguousarray(circ.gates)
     m = np.ascontiguousarray(circ.mats, np.complex128)
     cap = max(1, g.shape[0])
@@ -582,9 +582,9 @@ def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = F
         _check(lib().qsv_window_plan(g.ctypes.data_as(C.c_void_p), g.shape[0], mp, max_k, int(exact), tb,
                                      circ.n, *outs))
     else:
-        width, branch, first, delay = (int(v) for v in beam)
+        width, branch, first, delay, tries = (tuple(int(v) for v in beam) + (0,))[:5]
         _check(lib().qsv_window_plan_beam(g.ctypes.data_as(C.c_void_p), g.shape[0], mp, max_k, int(exact), tb,
-                                          circ.n, width, branch, first, delay, *outs))
+                                          circ.n, width, branch, first, delay, tries, *outs))
     out = []
     for b in range(nb.value):
         k = int(ks[b])

@@ -134,8 +134,8 @@ void planners(Rng &r, int rounds)
                 // the beam search over windows (the learned schedule's planner)
                 const int width = 1 + r.below(4), branch = 1 + r.below(3), first = r.below(7), delay = r.below(5);
                 const int rb = qsv_window_plan_beam(g.data(), count, mats.empty() ? nullptr : mats.data(), k, 0, tb,
-                                                    n, width, branch, first, delay, cap, kq.data(), qs.data(),
-                                                    m.data(), ng.data(), win.data(), &nb);
+                                                    n, width, branch, first, delay, r.below(9), cap, kq.data(),
+                                                    qs.data(), m.data(), ng.data(), win.data(), &nb);
                 check(rb == QSV_OK, "qsv_window_plan_beam");
                 total = 0;
                 for (int64_t b = 0; b < nb && rb == QSV_OK; b++) total += kq[b] ? ng[b] : 1;

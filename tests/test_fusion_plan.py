@@ -240,14 +240,18 @@ def test_beam_width_one_is_the_window_plan():
 
 @pytest.mark.parametrize("k", [4, 5])
 @pytest.mark.parametrize("tb", [7, 9, 13])
-def test_beam_plan_reproduces_random_circuit(k, tb):
+@pytest.mark.parametrize("tries", [0, 8])
+def test_beam_plan_reproduces_random_circuit(k, tb, tries):
+    """Beam windows, fused in the closure's, program or (tries > 0) random
+    dependency orders: the planned FP64 blocks reproduce the gate-by-gate
+    state."""
     n = 12
     circ = C.random_layers(n, 12, 700 + k + tb, haar_frac=0.2)
     psi = C.random_gaussian_state(n, 3 * k + tb)
     ref = O.State(n)
     ref.set(psi)
     ref.run(circ)
-    plan = E.window_plan(circ, k, tb, beam=BEAM)
+    plan = E.window_plan(circ, k, tb, beam=BEAM + (tries,))
     assert sum(ng for _, _, ng, _ in plan) == circ.ngates
     assert all(len(qs) <= k for qs, _, _, _ in plan)
     wins = {}
@@ -274,7 +278,7 @@ def test_beam_plan_every_gate_kind_and_measurements():
         if rec:
             b.add(*rec)
     circ = b.build()
-    plan = E.window_plan(circ, 4, 8, beam=BEAM)
+    plan = E.window_plan(circ, 4, 8, beam=BEAM + (8,))
     assert sum(p[2] for p in plan if p[0] is not None) == circ.ngates - 2
     # the barriers in order, each in a window of its own, after exactly the
     # records that precede them
@@ -294,3 +298,15 @@ def test_beam_needs_fewer_windows_on_the_benchmark_circuit():
     wb = len({w for *_, w in E.window_plan(circ, 4, 13, beam=BEAM)})
     w12 = len({w for *_, w in E.window_plan(circ, 4, 13, beam=(12, 4, 8, 6))})
     assert w12 <= 220 and wb <= 225 < w1, (w12, wb, w1)
+
+
+def test_random_fusion_orders_need_fewer_blocks():
+    """The learned schedule fuses each window in 24 random dependency orders
+    too: on the benchmark circuit's beam windows 1,092 blocks become <= 1,050
+    (1,027 measured), the same windows and records."""
+    circ = C.random_layers(30, 200, 2025, haar_frac=0.1)
+    a = E.window_plan(circ, 4, 13, beam=(12, 4, 8, 6, 0))
+    b = E.window_plan(circ, 4, 13, beam=(12, 4, 8, 6, 24))
+    assert len({w for *_, w in a}) == len({w for *_, w in b})
+    assert sum(p[2] for p in b) == circ.ngates
+    assert len(b) <= 1050 < len(a), (len(b), len(a))
