@@ -412,9 +412,10 @@ int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, i
  * closure's order and program order, and of `fuse_tries` random dependency
  * orders.  width = 1 and fuse_tries = 0 give qsv_window_plan's plan.  Outputs
  * as qsv_window_plan.  The learned scheduler (opts.scheduler = 1) plans this
- * way while the first windows run, with 24 fusion tries: width 12, branch 4,
- * first 8, delay 6 for >= 30 qubits, width 4, branch 3, first 6, delay 4 at
- * 29, width 1 below.  No reference counterpart. */
+ * way while the first windows run: width 12, branch 4, first 8, delay 6 for
+ * >= 30 qubits, width 4, branch 3, first 6, delay 4 at 29, width 1 below;
+ * 24 fusion tries from 29 qubits, 8 at 28, none below.  No reference
+ * counterpart. */
 int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
                          int tb, int n, int width, int branch, int first, int delay, int fuse_tries,
                          int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out, int64_t *ngates_out,

@@ -677,7 +677,10 @@ int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanIte
 // the model picked k = 5 for windows that ran slower)
 constexpr double kK5Margin = 0.10;
 // random dependency orders the learned schedule's fusion tries per window
-constexpr int kFuseTries = 24;
+// (~12 us each at k = 4 and 5 on ~40 records): 24 where a window's sweep
+// takes >= 2 ms (>= 29 qubits), 8 at 28 qubits (~1 ms), none below, so that
+// the planning thread stays ahead of the sweeps
+int fuse_tries(int n) { return n >= 29 ? 24 : n >= 28 ? 8 : 0; }
 
 struct ReadyWindow {
     int64_t barrier = -1;
@@ -802,8 +805,9 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
             // fusion searches random dependency orders too (seeded by the
             // window's first record, so plans repeat run to run)
             const uint64_t seed = (uint64_t)u.taken[0];
-            if ((rc = best_fusion_order(gates, u.taken, mats, 4, o.exact, kFuseTries, seed, order4, shape)) ||
-                (rc = best_fusion_order(gates, u.taken, mats, 5, o.exact, kFuseTries, seed, order5, shape5)))
+            const int tries = fuse_tries(s->n);
+            if ((rc = best_fusion_order(gates, u.taken, mats, 4, o.exact, tries, seed, order4, shape)) ||
+                (rc = best_fusion_order(gates, u.taken, mats, 5, o.exact, tries, seed, order5, shape5)))
                 return send_error(rc);
             const size_t n4 = shape.size();
             std::vector<int> win(n4, 0);
