@@ -144,6 +144,15 @@ int qsv_gather(qsv_state *s, const uint64_t *idx, int64_t count, double *amps);
  * comparing paths.  No reference counterpart. */
 int qsv_copy_state(qsv_state *dst, qsv_state *src);
 
+/* Moves every qubit q to position perm[q] (perm a permutation of 0..n-1): the
+ * amplitude of basis state x ends at the index whose bit perm[q] is bit q of
+ * x, for every q.  Pure data movement (bit-exact), in sweeps of 13-qubit tiles
+ * that hold qubits 0..3 (one sweep per 12 qubits a cycle of the permutation
+ * moves outside 0..3, at most ceil((n - 1) / 12) per cycle).  complex64 states
+ * of >= 13 qubits.  No reference counterpart: the window schedules relabel
+ * qubits this way between sweeps and restore the order at the end. */
+int qsv_permute_qubits(qsv_state *s, const int *perm);
+
 /* <a|b> in FP64 (either precision; same qubit count).  Fidelity of two states is
  * |<a|b>|^2 / (<a|a><b|b>).  No reference counterpart. */
 int qsv_inner(qsv_state *a, qsv_state *b, double *re, double *im);
@@ -336,6 +345,8 @@ typedef struct qsv_run_opts {
     int pass_tb;        /* scheduler 2: tile bits of such a pass (0 = minimal)  */
     int pass_kernel;    /* multi-block sweeps: 0 or 1 = one workgroup per tile  */
                         /* (other values: QSV_ERR_UNSUPPORTED)                  */
+    int fixed_low;      /* window schedules: 1 = qubits 0..3 stay in place (no  */
+                        /* relabelling between sweeps, see below)               */
 } qsv_run_opts;
 /* opts.scheduler: 0 = greedy blocks at fuse_k, one sweep each;
  *                 1 = learned (planner, block size, grouping into sweeps,
@@ -345,7 +356,15 @@ typedef struct qsv_run_opts {
  *                     tile bits holding qubits 0..3, every runnable gate
  *                     inside it fused into blocks of <= fuse_k (4, 5)
  *                     qubits, one sweep per window (states of >= 11 qubits;
- *                     smaller ones fall back to scheduler 0). */
+ *                     smaller ones fall back to scheduler 0).
+ * Relabelling (schedulers 1 and 3, >= 14 qubits, 13-bit tiles, unless
+ * fixed_low): a sweep's tile always holds the amplitudes' bits 0..3 (128-byte
+ * runs), but which qubits sit there changes: each window's sweep stores its
+ * tile with four of the window's qubits moved to bits 0..3 (those the next
+ * window starts from), so consecutive windows share four qubits instead of
+ * every window holding qubits 0..3.  Before a MEASURE / RESET record and at
+ * the end of the run the qubits return to their own bits (qsv_permute_qubits'
+ * sweeps), so the state leaves in the usual order. */
 
 /* Kernel classes for per-launch profiling. */
 enum {
@@ -410,16 +429,18 @@ int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, i
  * first growth candidates; a window is fixed once the best state is `delay`
  * windows past it.  Each window's records are fused in the better of the
  * closure's order and program order, and of `fuse_tries` random dependency
- * orders.  width = 1 and fuse_tries = 0 give qsv_window_plan's plan.  Outputs
- * as qsv_window_plan.  The learned scheduler (opts.scheduler = 1) plans this
+ * orders.  relabel != 0: each window grows from the four qubits the previous
+ * one leaves at bits 0..3 (see qsv_run_opts.fixed_low) instead of from qubits
+ * 0..3; the blocks keep their logical qubits.  width = 1, fuse_tries = 0 and
+ * relabel = 0 give qsv_window_plan's plan.  Outputs as qsv_window_plan.  The learned scheduler (opts.scheduler = 1) plans this
  * way while the first windows run: width 12, branch 4, first 8, delay 6 for
  * >= 30 qubits, width 4, branch 3, first 6, delay 4 at 29, width 1 below;
  * 24 fusion tries from 29 qubits, 8 at 28, none below.  No reference
  * counterpart. */
 int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
                          int tb, int n, int width, int branch, int first, int delay, int fuse_tries,
-                         int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out, int64_t *ngates_out,
-                         int *window_out, int64_t *nblocks);
+                         int relabel, int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                         int64_t *ngates_out, int *window_out, int64_t *nblocks);
 
 int qsv_run_circuit(qsv_state *s, const qsv_gate *gates, int64_t ngates,
                     const double *mats, const qsv_run_opts *opts,

@@ -599,6 +599,12 @@ int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanIte
     int rc = QSV_OK;
     for (const Launch &L : launches) {
         if (rc) break;
+        if (L.count == 0) {
+            if ((rc = prof->begin())) break;
+            rc = launch_pass_dev(s, {}, {}, 0, kMaxPassTileBits, 0, &L.moves);
+            if (!rc) rc = prof->end(QSV_K_PASS, 2 * 8 * N, 0);
+            continue;
+        }
         const PlanItem &it0 = plan[L.first];
         if (it0.barrier >= 0) {
             const qsv_gate &r = gates[it0.barrier];
@@ -633,7 +639,7 @@ int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanIte
         if (rng) *rng = qsv_pm_jump(*rng, (uint64_t)gates_here);
         if ((rc = prof->begin())) break;
         int cls = QSV_K_FUSED;
-        bool single = L.count == 1;
+        bool single = L.count == 1 && L.moves.empty();
         int tb = L.tb;
         if (single && !mfma[L.first]) {
             rc = launch_block(s, it0.qubits.data(), (int)it0.qubits.size(), it0.u.data(), 0);
@@ -643,7 +649,7 @@ int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanIte
             tb = (int)it0.qubits.size() + tc;
             rc = launch_fused_dev(s, it0.qubits.data(), (int)it0.qubits.size(), mp[0], o.split, tc);
         } else {
-            rc = launch_pass_dev(s, qsets, mp, o.split, tb, o.pass_kernel);
+            rc = launch_pass_dev(s, qsets, mp, o.split, tb, o.pass_kernel, L.moves.empty() ? nullptr : &L.moves);
             cls = QSV_K_PASS;
         }
         int64_t tag = -1;
@@ -765,6 +771,36 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
         bo.first = 6;
         bo.delay = 4;
     }
+    // relabelling: the window stream plans in logical qubits; phys[x] is the
+    // bit position of qubit x during the current window and at[p] the qubit
+    // at position p.  Each window's blocks are renamed to positions, and its
+    // last sweep stores the tile with the window's next_low qubits moved to
+    // positions 0..3 (exchanged with the qubits there).  Before a MEASURE /
+    // RESET and at the end, sweeps that only move qubits bring every qubit
+    // back to its own position.
+    const int n = s->n;
+    bo.relabel = !o.fixed_low && n > kMaxPassTileBits && (o.pass_tb == 0 || o.pass_tb == kMaxPassTileBits);
+    std::vector<int> phys(n), at(n);
+    for (int x = 0; x < n; x++) phys[x] = at[x] = x;
+    auto restore = [&]() {
+        bool moved = false;
+        for (int p = 0; p < n; p++) moved |= at[p] != p;
+        if (!moved) return;
+        ReadyWindow w;
+        for (auto &mv : qubit_routes(at, kMaxPassTileBits)) {
+            Launch L;
+            L.count = 0;
+            L.tb = kMaxPassTileBits;
+            L.moves = std::move(mv);
+            w.launches.push_back(std::move(L));
+        }
+        for (int x = 0; x < n; x++) phys[x] = at[x] = x;
+        q->push(std::move(w));
+    };
+    auto rename = [&](std::vector<PlanItem> &items) {
+        for (PlanItem &it : items)
+            for (int &x : it.qubits) x = phys[x];
+    };
     const int fixed_k = std::max(4, std::min(o.fuse_k, 5));
     WindowStream ws(gates, ngates, s->n, o.pass_tb ? o.pass_tb : kMaxPassTileBits, bo);
     std::vector<PlanItem> shape, shape5;
@@ -778,11 +814,13 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
         if (rc) return send_error(rc);
         ReadyWindow w;
         if (end) {
+            restore();
             w.end = true;
             q->push(std::move(w));
             return;
         }
         if (u.barrier >= 0) {
+            restore();
             w.barrier = u.barrier;
             PlanItem it;
             it.barrier = u.barrier;
@@ -799,6 +837,12 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
         // shapes, both in one batched inference) and takes k = 5 only for a
         // predicted gain above the model's resolution; then the blocks get
         // their matrices
+        if (bo.relabel) {
+            uint64_t low = 0;
+            for (int p = 0; p < 4; p++) low |= uint64_t(1) << at[p];
+            if (low != u.low || (u.low & ~u.mask) || (u.next_low & ~u.mask) || __builtin_popcountll(u.next_low) != 4)
+                return send_error(fail(QSV_ERR_ARG, "internal: relabelled window does not match the qubit order"));
+        }
         int k = fixed_k;
         double best = 0;
         if (learned) {
@@ -813,6 +857,7 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
             std::vector<int> win(n4, 0);
             win.resize(n4 + shape5.size(), 1);
             shape.insert(shape.end(), std::make_move_iterator(shape5.begin()), std::make_move_iterator(shape5.end()));
+            rename(shape);
             prepare_blocks(s, shape, win, shape_mfma);
             double total = 0;
             if ((rc = group_launches(s, shape, o, launches5, &total, &win))) return send_error(rc);
@@ -836,6 +881,7 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
         rc = learned ? fuse_order(gates, k == 5 ? order5 : order4, mats, k, o.exact, false, w.plan)
                      : fuse_window(gates, u.taken, mats, k, o.exact, false, w.plan);
         if (rc) return send_error(rc);
+        rename(w.plan);
         const std::vector<int> window(w.plan.size(), 0);
         prepare_blocks(s, w.plan, window, w.mfma);
         if (learned) {
@@ -845,6 +891,42 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
             w.launches = shape_launches;
         } else {
             window_launches(s, w.plan, window, w.mfma, o, w.launches);
+        }
+        if (bo.relabel && u.last) {
+            // the segment's last window starts putting qubits back: one pass
+            // of the restoring routes over its own tile's positions
+            uint64_t tile = 0;
+            for (uint64_t m = u.mask; m; m &= m - 1) tile |= uint64_t(1) << phys[__builtin_ctzll(m)];
+            Launch &L = w.launches.back();
+            L.moves = route_within(at, tile);
+            if (!L.moves.empty()) {
+                std::vector<int> nat(at);
+                for (const auto &mv : L.moves) nat[mv.second] = at[mv.first];
+                at.swap(nat);
+                for (int p = 0; p < n; p++) phys[at[p]] = p;
+                L.tb = kMaxPassTileBits;
+                L.tc = 0;
+            }
+        } else if (bo.relabel && u.next_low != u.low) {
+            // next_low's qubits move into the positions 0..3 whose qubits leave
+            int slot[4], ns = 0;
+            for (int p = 0; p < 4; p++)
+                if (!(u.next_low >> at[p] & 1)) slot[ns++] = p;
+            Launch &L = w.launches.back();
+            int si = 0;
+            for (uint64_t m = u.next_low; m; m &= m - 1) {
+                const int x = __builtin_ctzll(m);
+                if (phys[x] < 4) continue;
+                const int j = slot[si++], p = phys[x], y = at[j];
+                L.moves.push_back({p, j});
+                L.moves.push_back({j, p});
+                at[j] = x;
+                at[p] = y;
+                phys[x] = j;
+                phys[y] = p;
+            }
+            L.tb = kMaxPassTileBits;
+            L.tc = 0;
         }
         w.k = k;
         w.pred = best;

@@ -52,14 +52,18 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // VGPRs); else 256 (the k = 5 product needs up to 210 VGPRs).
 __host__ __device__ constexpr int pass_threads(int tb, bool any5) { return tb == 13 && !any5 ? 512 : 256; }
 
-// Bank bits 1..4 XOR-ed into the LDS float index by bits 5, 6, 7, 8.
-constexpr int kLdsSwizzle[4] = {0x1E, 0x12, 0x14, 0x1C};
+// Bank bits 1..4 XOR-ed into the LDS float index by bits 5..12.  The 32
+// columns of a block's half-wave access are the 5 lowest tile bits outside
+// the block (all below bit 9), so bits 9..12 only rotate a block's banks; they
+// spread the lanes of a permuted store (k_fused_pass with PERM), whose lanes
+// may read elements that differ in those bits alone.
+constexpr int kLdsSwizzle[8] = {0x1E, 0x12, 0x14, 0x1C, 0x06, 0x0C, 0x18, 0x0A};
 
-__host__ __device__ inline int lds_swizzle(int e)
+__host__ __device__ constexpr int lds_swizzle(int e)
 {
     int s = 0;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 8; i++)
         if (e >> (5 + i) & 1) s ^= kLdsSwizzle[i];
     return e ^ s;
 }
@@ -80,6 +84,16 @@ struct PassArgs {
     InsertSet tile_ins;     // L positions, ascending
     int lpos[16];           // compact tile bit j -> qubit
     uint64_t hioff[16];     // pdep(2 NT i, L): address part of load i (NT = threads that move the tile)
+    // permuted store (PERM): the amplitudes leave with some of the tile's
+    // qubits exchanged.  Thread tid's store i writes the output elements
+    // o and o + 1 at HBM offset (OR of st_hbit[j] over tid's set bits j) +
+    // st_hioff[i], read from the LDS floats x and x ^ st_pair, x = (XOR of
+    // st_lbit[j] over tid's set bits j) ^ st_li[i]
+    uint64_t st_hbit[9];
+    uint64_t st_hioff[16];
+    int st_lbit[9];
+    int st_li[16];
+    int st_pair;
     int nblocks;
     float bscale, binv;     // FP16x3: state scale 2^s and 2^-s
     float one;              // 1.0f (see fp16x3_product)
@@ -481,7 +495,7 @@ __device__ __forceinline__ void pass_apply4_cols(float *pre, const PassBlock &pb
 
 // ANY5: some block of the pass has 5 qubits (otherwise only the k = 4 product
 // is compiled in, which keeps the register count at the k = 4 level).
-template <int TB, int MODE, bool ANY5>
+template <int TB, int MODE, bool ANY5, bool PERM>
 __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t ntiles, const PassArgs &pa)
 {
     constexpr int TSIZE = 1 << TB;
@@ -503,8 +517,9 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     float4 buf[NLD];
 #pragma unroll
     for (int i = 0; i < NLD; i++) buf[i] = *reinterpret_cast<const float4 *>(a + base + toff + pa.hioff[i]);
-    // elements 2 tid + 2 NT i: bits 5..8 come from tid alone, so the swizzle is
-    // one XOR per thread; it keeps bit 0, so element + 1 follows at index + 1
+    // elements 2 tid + 2 NT i: the swizzle is linear, so it is one XOR for
+    // tid's bits and a constant per (unrolled) i; it keeps bit 0, so element +
+    // 1 follows at index + 1
     const int sw = lds_swizzle(2 * tid) ^ (2 * tid);
     // FP16x3: the tile sits in LDS scaled by 2^s (exact) from staging to the
     // store, so the blocks split it without a multiply on either side
@@ -513,7 +528,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     // accesses a half-wave's even addresses would collide 2-way in the banks
 #pragma unroll
     for (int i = 0; i < NLD; i++) {
-        const int e = (2 * (tid + NT * i)) ^ sw;
+        const int e = (2 * (tid + NT * i)) ^ sw ^ (lds_swizzle(2 * NT * i) ^ (2 * NT * i));
         *reinterpret_cast<float2 *>(pre + e) = make_float2(buf[i].x * ls, buf[i].z * ls);
         *reinterpret_cast<float2 *>(pim + e) = make_float2(buf[i].y * ls, buf[i].w * ls);
     }
@@ -538,23 +553,42 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
             __syncthreads();
         }
     }
+    if constexpr (PERM) {
+        uint64_t soff = 0;
+        int sl = 0;
 #pragma unroll
-    for (int i = 0; i < NLD; i++) {
-        const int e = (2 * (tid + NT * i)) ^ sw;
-        const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
-        float4 v;
-        v.x = r.x * li; v.y = m.x * li;
-        v.z = r.y * li; v.w = m.y * li;
-        *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
+        for (int j = 0; j < TBITS; j++)
+            if ((tid >> j) & 1) {
+                soff |= pa.st_hbit[j];
+                sl ^= pa.st_lbit[j];
+            }
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int e0 = sl ^ pa.st_li[i], e1 = e0 ^ pa.st_pair;
+            float4 v;
+            v.x = pre[e0] * li; v.y = pim[e0] * li;
+            v.z = pre[e1] * li; v.w = pim[e1] * li;
+            *reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]) = v;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            const int e = (2 * (tid + NT * i)) ^ sw ^ (lds_swizzle(2 * NT * i) ^ (2 * NT * i));
+            const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
+            float4 v;
+            v.x = r.x * li; v.y = m.x * li;
+            v.z = r.y * li; v.w = m.y * li;
+            *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
+        }
     }
 }
 
 static_assert(sizeof(PassArgs) <= 4096, "pass arguments must fit the 4 KiB kernel-argument budget");
 
-template <int TB, int MODE, bool ANY5>
+template <int TB, int MODE, bool ANY5, bool PERM>
 __global__ __launch_bounds__(pass_threads(TB, ANY5)) void k_fused_pass(float2 *__restrict__ a, uint64_t ntiles, PassArgs pa)
 {
-    fused_pass_body<TB, MODE, ANY5>(a, ntiles, pa);
+    fused_pass_body<TB, MODE, ANY5, PERM>(a, ntiles, pa);
 }
 
 // Tile bits a pass over these (widened) blocks needs: qubits 0..3 (the
@@ -569,34 +603,109 @@ int pass_tile_bits(const std::vector<const std::vector<int> *> &qsets, int n)
     return tb <= std::min(kMaxPassTileBits, n) ? tb : -1;
 }
 
-template <int TB, bool ANY5>
+template <int TB, bool ANY5, bool PERM>
 static void launch_tb5(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split)
 {
     float2 *a = (float2 *)s->amps;
     const dim3 g((unsigned)ntiles), t(pass_threads(TB, ANY5));
-    if (split == 3) hipLaunchKernelGGL((k_fused_pass<TB, 3, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
-    else if (split == 2) hipLaunchKernelGGL((k_fused_pass<TB, 2, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
-    else if (split) hipLaunchKernelGGL((k_fused_pass<TB, 1, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
-    else hipLaunchKernelGGL((k_fused_pass<TB, 0, ANY5>), g, t, 0, s->stream, a, ntiles, pa);
+    if (split == 3) hipLaunchKernelGGL((k_fused_pass<TB, 3, ANY5, PERM>), g, t, 0, s->stream, a, ntiles, pa);
+    else if (split == 2) hipLaunchKernelGGL((k_fused_pass<TB, 2, ANY5, PERM>), g, t, 0, s->stream, a, ntiles, pa);
+    else if (split) hipLaunchKernelGGL((k_fused_pass<TB, 1, ANY5, PERM>), g, t, 0, s->stream, a, ntiles, pa);
+    else hipLaunchKernelGGL((k_fused_pass<TB, 0, ANY5, PERM>), g, t, 0, s->stream, a, ntiles, pa);
 }
 
+// perm: the pass exchanges qubits on the way out (13-bit tiles only)
 template <int TB>
-static void launch_tb(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split)
+static void launch_tb(qsv_state *s, uint64_t ntiles, const PassArgs &pa, int split, bool perm)
 {
     bool any5 = false;
     for (int g = 0; g < pa.nblocks; g++) any5 |= pa.b[g].kq == 5;
-    if (any5) launch_tb5<TB, true>(s, ntiles, pa, split);
-    else launch_tb5<TB, false>(s, ntiles, pa, split);
+    if constexpr (TB == kMaxPassTileBits) {
+        if (perm) {
+            if (any5) launch_tb5<TB, true, true>(s, ntiles, pa, split);
+            else launch_tb5<TB, false, true>(s, ntiles, pa, split);
+            return;
+        }
+    }
+    if (any5) launch_tb5<TB, true, false>(s, ntiles, pa, split);
+    else launch_tb5<TB, false, false>(s, ntiles, pa, split);
+}
+
+// The permuted store's thread layout.  tau[j] = the input tile bit whose
+// qubit leaves at output tile bit j.  Output bits 1..3 are thread bits 0..2
+// (eight lanes store one 128-byte run); thread bits 3..5 take the three of
+// output bits 4..8 that spread a wave's LDS reads over the most banks (32
+// banks per half-wave and 64 per wave, both scored), so that a wave's stores
+// stay within 4 KiB; the remaining bits follow in ascending order.  (Lane bits
+// taken from the whole tile, which may hold qubits 2^26 amplitudes apart, made
+// a pass that exchanges qubits 0..3 with 26..29 take 4.65 ms at 30 qubits
+// against 3.1-3.4 ms for other exchanges.)
+static void permuted_store(PassArgs &pa, int tb, int nt, const int *tau)
+{
+    const int tbits = __builtin_ctz(nt);
+    auto q = [&](int j) { return lds_swizzle(1 << tau[j]); };
+    int ob[16], nob = 0;
+    for (int j = 1; j <= 3; j++) ob[nob++] = j;
+    int best[3] = {4, 5, 6}, best_score = 1 << 30;
+    const int top = std::min(tb, 9);
+    for (int x = 4; x < top; x++)
+        for (int y = x + 1; y < top; y++)
+            for (int z = 4; z < top; z++) {
+                if (z == x || z == y) continue;
+                const int lb[6] = {1, 2, 3, x, y, z};
+                int c32[2][32] = {}, c64[64] = {}, m32 = 0, m64 = 0;
+                for (int l = 0; l < 64; l++) {
+                    int e = 0;
+                    for (int b = 0; b < 6; b++)
+                        if (l >> b & 1) e ^= q(lb[b]);
+                    m32 = std::max(m32, ++c32[l >> 5][e & 31]);
+                    m64 = std::max(m64, ++c64[e & 63]);
+                }
+                const int score = 64 * m32 + m64;
+                if (score < best_score) {
+                    best_score = score;
+                    best[0] = x;
+                    best[1] = y;
+                    best[2] = z;
+                }
+            }
+    for (int b = 0; b < 3; b++) ob[nob++] = best[b];
+    for (int j = 4; j < tb; j++)
+        if (j != best[0] && j != best[1] && j != best[2]) ob[nob++] = j;
+    // ob[0 .. tbits) are thread bits, the rest store-index bits
+    for (int b = 0; b < tbits; b++) {
+        pa.st_hbit[b] = uint64_t(1) << pa.lpos[ob[b]];
+        pa.st_lbit[b] = q(ob[b]);
+    }
+    const int nld = (1 << tb) / (2 * nt);
+    for (int i = 0; i < nld; i++) {
+        uint64_t h = 0;
+        int l = 0;
+        for (int c = 0; tbits + c < tb - 1; c++)
+            if (i >> c & 1) {
+                h |= uint64_t(1) << pa.lpos[ob[tbits + c]];
+                l ^= q(ob[tbits + c]);
+            }
+        pa.st_hioff[i] = h;
+        pa.st_li[i] = l;
+    }
+    pa.st_pair = q(0);
 }
 
 // blocks[g]: widened qubit list (4 or 5 qubits) and device operand pointer.
 // kernel: 0 or 1 (one workgroup per tile, the only sweep kernel).
+// moves: (from, to) qubit pairs: the amplitudes leave the pass with the
+// qubit at `from` moved to `to` (a permutation of some of the tile's qubits;
+// 13-bit tiles only; a pass that only moves qubits has no blocks).
 int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
-                    const std::vector<const float *> &mops, int split, int tb, int kernel)
+                    const std::vector<const float *> &mops, int split, int tb, int kernel,
+                    const std::vector<std::pair<int, int>> *moves)
 {
     if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "fused passes need a c64 state");
     const int G = (int)qsets.size();
-    if (G < 1 || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..16 blocks per pass");
+    const bool perm = moves && !moves->empty();
+    if (G < (perm ? 0 : 1) || G > kMaxPassBlocks) return fail(QSV_ERR_ARG, "1..16 blocks per pass");
+    if (perm && tb != kMaxPassTileBits) return fail(QSV_ERR_ARG, "passes that move qubits have 13-bit tiles");
     if (tb < 11 || tb > kMaxPassTileBits || tb > s->n) return fail(QSV_ERR_ARG, "pass tile bits must be 11..13");
     if (split < 0 || split > 3) return fail(QSV_ERR_ARG, "split must be 0, 1, 2 or 3");
     if (kernel < 0 || kernel > 1) return fail(QSV_ERR_UNSUPPORTED, "pass kernel must be 0 or 1");
@@ -608,6 +717,17 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
             u |= uint64_t(1) << x;
         }
     }
+    uint64_t from = 0, to = 0;
+    if (perm)
+        for (const auto &m : *moves) {
+            if (m.first < 0 || m.first >= s->n || m.second < 0 || m.second >= s->n)
+                return fail(QSV_ERR_ARG, "moved qubit out of range");
+            if ((from >> m.first & 1) || (to >> m.second & 1)) return fail(QSV_ERR_ARG, "qubit moved twice");
+            from |= uint64_t(1) << m.first;
+            to |= uint64_t(1) << m.second;
+        }
+    if (from != to) return fail(QSV_ERR_ARG, "qubit moves are not a permutation");
+    u |= from;
     if (4 + __builtin_popcountll(u & ~uint64_t(15)) > tb) return fail(QSV_ERR_ARG, "blocks do not fit the pass tile");
     uint64_t lmask = u;
     for (int b = 0; __builtin_popcountll(lmask) < tb; b++) lmask |= uint64_t(1) << b;
@@ -634,6 +754,17 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
             if (lpos[i] == bit) return i;
         return -1;
     };
+    if (perm) {
+        // tau[output tile bit] = input tile bit
+        int tau[16];
+        for (int i = 0; i < tb; i++) {
+            int dst = lpos[i];
+            for (const auto &m : *moves)
+                if (m.first == lpos[i]) dst = m.second;
+            tau[rank(dst)] = i;
+        }
+        permuted_store(pa, tb, nt, tau);
+    }
     pa.nblocks = G;
     for (int g = 0; g < G; g++) {
         PassBlock &pb = pa.b[g];
@@ -677,9 +808,9 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
     }
     const uint64_t ntiles = s->dim >> tb;
     switch (tb) {
-    case 11: launch_tb<11>(s, ntiles, pa, split); break;
-    case 12: launch_tb<12>(s, ntiles, pa, split); break;
-    default: launch_tb<13>(s, ntiles, pa, split); break;
+    case 11: launch_tb<11>(s, ntiles, pa, split, false); break;
+    case 12: launch_tb<12>(s, ntiles, pa, split, false); break;
+    default: launch_tb<13>(s, ntiles, pa, split, perm); break;
     }
     s->launches++;
     QSV_HIP(hipGetLastError());
@@ -687,3 +818,25 @@ int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &q
 }
 
 }  // namespace qsv
+
+using namespace qsv;
+
+extern "C" int qsv_permute_qubits(qsv_state *s, const int *perm)
+{
+    QSV_CHECK_STATE(s);
+    if (!perm) return fail(QSV_ERR_ARG, "null permutation");
+    if (s->prec != QSV_C64) return fail(QSV_ERR_PRECISION, "qubit permutation runs on c64 states");
+    if (s->n < kMaxPassTileBits) return fail(QSV_ERR_UNSUPPORTED, "qubit permutation needs >= 13 qubits");
+    std::vector<int> dest(s->n);
+    uint64_t hit = 0;
+    for (int q = 0; q < s->n; q++) {
+        if (perm[q] < 0 || perm[q] >= s->n || (hit >> perm[q] & 1)) return fail(QSV_ERR_ARG, "not a permutation");
+        hit |= uint64_t(1) << perm[q];
+        dest[q] = perm[q];
+    }
+    for (const auto &moves : qubit_routes(dest, kMaxPassTileBits)) {
+        const int rc = launch_pass_dev(s, {}, {}, 0, kMaxPassTileBits, 0, &moves);
+        if (rc) return rc;
+    }
+    return QSV_OK;
+}

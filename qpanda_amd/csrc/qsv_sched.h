@@ -64,12 +64,17 @@ struct WindowUnit {
     int64_t barrier = -1;              // record index of a MEASURE / RESET, else -1
     uint64_t mask = 0;                 // the window's qubits
     std::vector<int64_t> taken;        // its records, in a dependency order
+    uint64_t low = 0;                  // the qubits at bits 0..3 during the sweep
+    uint64_t next_low = 0;             // the qubits the sweep leaves at bits 0..3
+    bool last = false;                 // the last window before a MEASURE / RESET or the end
 };
 struct BeamOpts {
     int width = 1;                     // states kept per depth (1: plan_window's windows)
     int branch = 3;                    // windows expanded per state
     int first = 6;                     // first growth candidates completed per state
     int delay = 4;                     // windows a commitment trails the search
+    bool relabel = false;              // windows grow from the qubits the previous
+                                       // one left at bits 0..3 (else from 0..3)
 };
 class WindowStream {
 public:
@@ -102,22 +107,33 @@ size_t fused_operand_floats(int kq, int split);
 constexpr int kDefaultTileCols = 7;
 int launch_fused_dev(qsv_state *s, const int *qubits, int kq, const float *mops, int split, int tc);
 
+// Passes that move the data at position p to position dest[p] for every p
+// (dest a permutation of the positions 0..n-1): each pass exchanges the qubits
+// of one tile of tb positions that holds 0..3, given as (from, to) pairs
+// (qsv_window.cpp).
+std::vector<std::vector<std::pair<int, int>>> qubit_routes(std::vector<int> dest, int tb);
+// One such pass over the positions in mask `in` (which holds 0..3): the moves
+// that put the most of them in place.
+std::vector<std::pair<int, int>> route_within(const std::vector<int> &dest, uint64_t in);
+
 // Multi-block passes (qsv_pass.hip).
 constexpr int kMaxPassBlocks = 16;
 constexpr int kMaxPassTileBits = 13;
 int pass_tile_bits(const std::vector<const std::vector<int> *> &qsets, int n);
 int launch_pass_dev(qsv_state *s, const std::vector<const std::vector<int> *> &qsets,
-                    const std::vector<const float *> &mops, int split, int tb, int kernel = 0);
+                    const std::vector<const float *> &mops, int split, int tb, int kernel = 0,
+                    const std::vector<std::pair<int, int>> *moves = nullptr);
 
 // One device launch of a fused run: plan items [first, first + count) (all
 // blocks) in one sweep.  count == 1 uses the single-block kernel with tile
 // column bits `tc`; count > 1 uses the pass kernel with `tb` tile bits.
 struct Launch {
     size_t first = 0;
-    int count = 1;
+    int count = 1;                     // 0: a sweep that only moves qubits
     int tc = 0;
     int tb = 0;
     double pred_ms = 0;
+    std::vector<std::pair<int, int>> moves;   // (from, to) qubit moves of the sweep's store
 };
 
 // Cost-model features of one launch (single block: tb = kq + tc).

@@ -242,12 +242,33 @@ struct Segment {
 
     uint64_t grow(int tb, int look) const { return grow_from(base_window(), head.data(), tb, look); }
 
-    // The starting windows of a beam state's expansion from positions
-    // `from`: the base window (whose lookahead growth is the heuristic's own
-    // window) and the base plus each of its `first` best first candidates.
-    void starts(const size_t *from, int tb, int first, std::vector<uint64_t> &out) const
+    // Relabelling: the four qubits window w leaves at bits 0..3 for the next
+    // window, which must grow from them.  The next window is estimated by a
+    // growth from nothing (positions `from`, after w); the four are taken
+    // from its qubits that w holds -- first those already at bits 0..3 (no
+    // data moves for them), then the others -- and then from w's other
+    // qubits.  With nothing left to run, the qubits stay.  On the 30-qubit
+    // depth-200 benchmark circuit the lookahead planner needs 187 windows this
+    // way against 234 with qubits 0..3 in every window.
+    uint64_t next_low(const size_t *from, uint64_t w, uint64_t low, int tb, int look) const
     {
-        const uint64_t w0 = base_window();
+        bool any = false;
+        for (int q = 0; q < n && !any; q++) any = from[q] < queue[q].size();
+        if (!any) return low;
+        const uint64_t f = grow_from(0, from, tb, look);
+        const uint64_t order[4] = {low & f, f & w & ~low, low & ~f, w & ~f & ~low};
+        uint64_t out = 0;
+        int c = 0;
+        for (uint64_t m : order)
+            for (; m && c < 4; m &= m - 1, c++) out |= m & (~m + 1);
+        return out;
+    }
+
+    // The starting windows of a beam state's expansion from positions
+    // `from`: the base window w0 (whose lookahead growth is the heuristic's
+    // own window) and the base plus each of its `first` best first candidates.
+    void starts(const size_t *from, uint64_t w0, int tb, int first, std::vector<uint64_t> &out) const
+    {
         out.assign(1, w0);
         size_t h[64];
         std::copy(from, from + n, h);
@@ -565,19 +586,28 @@ struct WindowStream::Impl {
     std::vector<size_t> committed;     // queue positions after the committed windows
     int64_t ncommit = 0;               // windows committed in the active segment
     int64_t total = 0;                 // windows committed in all
+    uint64_t clow = 0;                 // qubits at bits 0..3 after the committed windows
+    struct Step {
+        uint64_t w, next_low;
+    };
     struct State {
         std::vector<size_t> head;
         int64_t done = 0;
-        std::vector<uint64_t> path;    // windows not yet committed
+        uint64_t low = 0;              // qubits at bits 0..3 after the path
+        std::vector<Step> path;        // windows not yet committed
     };
     std::vector<State> beam;
     std::deque<WindowUnit> out;
     std::unique_ptr<JobPool> pool;
 
-    void commit(uint64_t w)
+    void commit(const Step &st)
     {
+        const uint64_t w = st.w;
         WindowUnit u;
         u.mask = w;
+        u.low = clow;
+        u.next_low = st.next_low ? st.next_low : clow;
+        clow = u.next_low;
         std::vector<int64_t> t;
         sg.closure(w, committed.data(), &t);
         u.taken.reserve(t.size());
@@ -597,8 +627,10 @@ struct WindowStream::Impl {
         next_rec = j;
         committed.assign(n, 0);
         ncommit = 0;
+        clow = sg.base_window();
         beam.assign(1, State());
         beam[0].head.assign(n, 0);
+        beam[0].low = clow;
         active = true;
         return QSV_OK;
     }
@@ -616,9 +648,10 @@ struct WindowStream::Impl {
         std::vector<Job> jobs;
         std::vector<uint64_t> ws;
         for (size_t i = 0; i < beam.size(); i++) {
-            if (b.width <= 1) ws.assign(1, sg.base_window());
-            else sg.starts(beam[i].head.data(), tb, b.first, ws);
-            for (uint64_t w0 : ws) jobs.push_back({i, w0});
+            const uint64_t w0 = b.relabel ? beam[i].low : sg.base_window();
+            if (b.width <= 1) ws.assign(1, w0);
+            else sg.starts(beam[i].head.data(), w0, tb, b.first, ws);
+            for (uint64_t w0s : ws) jobs.push_back({i, w0s});
         }
         if (b.width > 1 && !pool) {
             const int hw = (int)std::thread::hardware_concurrency();
@@ -651,7 +684,7 @@ struct WindowStream::Impl {
                 const int64_t got = sg.closure(w, c.head.data(), nullptr);
                 if (got == 0) continue;
                 c.done += got;
-                c.path.push_back(w);
+                c.path.push_back({w, 0});
                 uint64_t hsh = 1469598103934665603ull;
                 for (size_t x : c.head) hsh = (hsh ^ x) * 1099511628211ull;
                 if (!seen.insert(hsh).second) continue;
@@ -661,19 +694,32 @@ struct WindowStream::Impl {
         if (next.empty()) return fail(QSV_ERR_ARG, "window planner made no progress");
         std::stable_sort(next.begin(), next.end(), [](const State &x, const State &y) { return x.done > y.done; });
         if (next[0].done == sg.count) {
-            for (uint64_t w : next[0].path) commit(w);
+            for (const Step &st : next[0].path) commit(st);
+            out.back().last = true;
             beam.clear();
             active = false;
             return QSV_OK;
         }
         if ((int)next.size() > b.width) next.resize(b.width);
+        if (b.relabel) {
+            // the survivors' next starting qubits, on the pool
+            auto choose = [&](size_t j) {
+                State &c = next[j];
+                // c.low is still the low the window grew from (its parent's)
+                c.low = sg.next_low(c.head.data(), c.path.back().w, c.low, tb, kLookahead);
+                c.path.back().next_low = c.low;
+            };
+            if (pool) pool->run(next.size(), choose);
+            else
+                for (size_t j = 0; j < next.size(); j++) choose(j);
+        }
         const size_t delay = b.width <= 1 ? 0 : (size_t)std::min<int64_t>(b.delay, ncommit + 1);
         while (next[0].path.size() > delay) {
-            const uint64_t w = next[0].path[0];
-            commit(w);
+            const Step s0 = next[0].path[0];
+            commit(s0);
             std::vector<State> keep;
             for (State &st : next)
-                if (st.path[0] == w) {
+                if (st.path[0].w == s0.w && st.path[0].next_low == s0.next_low) {
                     st.path.erase(st.path.begin());
                     keep.push_back(std::move(st));
                 }
@@ -748,6 +794,79 @@ std::vector<int> window_widening(const std::vector<PlanItem> &plan, const std::v
     return add;
 }
 
+std::vector<std::pair<int, int>> route_within(const std::vector<int> &dest, uint64_t in)
+{
+    // every position whose data belongs at another position of the set sends
+    // it there; the last position of each chain (whose data belongs outside)
+    // sends its data to the chain's first position
+    const int n = (int)dest.size();
+    std::vector<int> src(n);
+    for (int p = 0; p < n; p++) src[dest[p]] = p;
+    auto inside = [&](int p) { return (in >> p & 1) != 0; };
+    std::vector<std::pair<int, int>> moves;
+    for (int p = 0; p < n; p++) {
+        if (!inside(p) || dest[p] == p) continue;
+        int to = dest[p];
+        if (!inside(to)) {
+            to = p;
+            while (inside(src[to]) && src[to] != p) to = src[to];
+        }
+        if (to != p) moves.push_back({p, to});
+    }
+    return moves;
+}
+
+std::vector<std::vector<std::pair<int, int>>> qubit_routes(std::vector<int> dest, int tb)
+{
+    // each pass takes the positions 0..3 and walks the cycles of p -> dest[p]
+    // (those through 0..3 first) until the tile is full; inside it every
+    // position whose data belongs at another position of the tile sends it
+    // there, and the last position of each chain (whose data belongs outside)
+    // sends its data to the chain's first position.  A chain of m positions
+    // puts m - 1 of them in place, so a cycle of c positions outside 0..3
+    // takes ceil((c - 1) / (tb - 1)) passes.
+    const int n = (int)dest.size();
+    std::vector<std::vector<std::pair<int, int>>> out;
+    std::vector<char> in(n), seen(n);
+    for (;;) {
+        bool done = true;
+        for (int p = 0; p < n; p++) done &= dest[p] == p;
+        if (done) return out;
+        std::fill(in.begin(), in.end(), 0);
+        std::fill(seen.begin(), seen.end(), 0);
+        int cnt = 0;
+        auto add = [&](int p) {
+            if (in[p]) return true;
+            if (cnt >= tb) return false;
+            in[p] = 1;
+            cnt++;
+            return true;
+        };
+        for (int p = 0; p < std::min(4, n); p++) add(p);
+        auto walk = [&](int p0) {
+            int p = p0;
+            do {
+                seen[p] = 1;
+                if (!add(p)) break;
+                p = dest[p];
+            } while (p != p0 && !seen[p]);
+        };
+        for (int p = 0; p < std::min(4, n); p++)
+            if (!seen[p] && dest[p] != p) walk(p);
+        for (int p = 4; p < n && cnt < tb; p++)
+            if (!seen[p] && dest[p] != p) walk(p);
+        uint64_t mask = 0;
+        for (int p = 0; p < n; p++)
+            if (in[p]) mask |= uint64_t(1) << p;
+        std::vector<std::pair<int, int>> moves = route_within(dest, mask);
+        if (moves.empty()) return {};       // cannot happen: a walk adds a position and its successor
+        std::vector<int> nd(dest);
+        for (const auto &m : moves) nd[m.second] = dest[m.first];
+        dest.swap(nd);
+        out.push_back(std::move(moves));
+    }
+}
+
 }  // namespace qsv
 
 using namespace qsv;
@@ -775,7 +894,7 @@ static int emit_plan(const std::vector<PlanItem> &plan, const std::vector<int> &
 
 extern "C" int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,
                                     int tb, int n, int width, int branch, int first, int delay, int fuse_tries,
-                                    int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
+                                    int relabel, int64_t max_blocks, int *k_out, int *qubits_out, double *mats_out,
                                     int64_t *ngates_out, int *window_out, int64_t *nblocks)
 {
     if ((!gates && ngates) || !nblocks) return fail(QSV_ERR_ARG, "null argument");
@@ -790,6 +909,7 @@ extern "C" int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const
     bo.branch = branch;
     bo.first = first;
     bo.delay = delay;
+    bo.relabel = relabel != 0;
     WindowStream ws(gates, ngates, n, tb, bo);
     std::vector<PlanItem> plan, blocks;
     std::vector<int> window;

@@ -29,7 +29,7 @@ EXPORTED = [
     "qsv_marginals", "qsv_pmeasure", "qsv_expect_zz_chain", "qsv_run_circuit",
     "qsv_sample_shots", "qsv_sample_shots_copy", "qsv_gather", "qsv_pm_next", "qsv_pm_jump", "qsv_pm_seed_from_tm", "qsv_version",
     "qsv_last_error", "qsv_device_count", "qsv_copy_state", "qsv_inner", "qsv_fuse_plan", "qsv_window_plan",
-    "qsv_window_plan_beam",
+    "qsv_window_plan_beam", "qsv_permute_qubits",
     "qsv_cm_create", "qsv_cm_destroy", "qsv_cm_nparams", "qsv_cm_get_params", "qsv_cm_set_params",
     "qsv_cm_block_features", "qsv_cm_forward", "qsv_cm_gradients", "qsv_cm_train", "qsv_set_costmodel",
     "qsv_timing_log", "qsv_batch_create", "qsv_batch_destroy", "qsv_batch_autotune", "qsv_batch_run",
@@ -47,7 +47,8 @@ class QsvError(RuntimeError):
 class RunOpts(C.Structure):
     _fields_ = [("exact", C.c_int), ("fuse_k", C.c_int), ("split", C.c_int),
                 ("scheduler", C.c_int), ("profile", C.c_int), ("tile", C.c_int),
-                ("group", C.c_int), ("pass_tb", C.c_int), ("pass_kernel", C.c_int)]
+                ("group", C.c_int), ("pass_tb", C.c_int), ("pass_kernel", C.c_int),
+                ("fixed_low", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -138,7 +139,7 @@ def lib():
                                C.POINTER(C.c_int64)], I),
             "qsv_window_plan": ([vp, C.c_int64, vp, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
                                  C.POINTER(C.c_int64)], I),
-            "qsv_window_plan_beam": ([vp, C.c_int64, vp, I, I, I, I, I, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
+            "qsv_window_plan_beam": ([vp, C.c_int64, vp, I, I, I, I, I, I, I, I, I, I, C.c_int64, vp, vp, vp, vp, vp,
                                       C.POINTER(C.c_int64)], I),
             "qsv_cm_create": ([U64, C.POINTER(vp)], I),
             "qsv_cm_destroy": ([vp], I),
@@ -158,6 +159,7 @@ def lib():
             "qsv_batch_graph_run": ([vp, I, vp, C.POINTER(D)], I),
             "qsv_copy_state": ([vp, vp], I),
             "qsv_inner": ([vp, vp, C.POINTER(D), C.POINTER(D)], I),
+            "qsv_permute_qubits": ([vp, C.POINTER(I)], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -257,6 +259,13 @@ class StateVector:
 
     def copy_from(self, other: "StateVector"):
         _check(lib().qsv_copy_state(self.h, other.h))
+
+    def permute_qubits(self, perm) -> None:
+        """Move qubit q to position perm[q] (complex64, >= 13 qubits; bit-exact data movement)."""
+        p = np.ascontiguousarray(perm, dtype=np.int32)
+        if p.shape != (self.n,):
+            raise ValueError("perm must list a position for every qubit")
+        _check(lib().qsv_permute_qubits(self.h, p.ctypes.data_as(C.POINTER(C.c_int))))
 
     def inner(self, other: "StateVector") -> complex:
         re, im = C.c_double(), C.c_double()
@@ -361,15 +370,16 @@ class StateVector:
 
     def run(self, circ: "_circ.Circuit", exact=False, fuse_k=0, split=2, scheduler=0,
             rng_state=None, outcomes=None, profile=False, tile=0, group=0, pass_tb=0,
-            pass_kernel=0) -> RunStats:
+            pass_kernel=0, fixed_low=False) -> RunStats:
         """scheduler: 0 greedy blocks (one sweep each), 1 learned (needs an attached
         cost model), 2 greedy blocks grouped into multi-block sweeps of <= group,
         3 window planner (one sweep per window of pass_tb tile bits, 0 = 13).
-        pass_kernel: 0 or 1 (one workgroup per tile)."""
+        pass_kernel: 0 or 1 (one workgroup per tile).  fixed_low: window
+        schedules keep qubits 0..3 in every sweep (no relabelling)."""
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
         opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile),
-                       int(group), int(pass_tb), int(pass_kernel))
+                       int(group), int(pass_tb), int(pass_kernel), int(fixed_low))
         st = RunStats()
         rs = C.c_int32(rng_state if rng_state is not None else 1)
         self._last_rng = None
@@ -565,7 +575,8 @@ def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = F
     """Window plan (host code; no device needed): list of (qubits, matrix,
     ngates, window) blocks, or (None, None, record_index, window) for
     MEASURE/RESET records, in emission order.  beam = (width, branch, first,
-    delay[, fuse_tries]) plans with the beam search (qsv_window_plan_beam)."""
+    delay[, fuse_tries[, relabel]]) plans with the beam search
+    (qsv_window_plan_beam)."""
     g = np.ascontiguousarray(circ.gates)
     m = np.ascontiguousarray(circ.mats, np.complex128)
     cap = max(1, g.shape[0])
@@ -582,9 +593,9 @@ def window_plan(circ: "_circ.Circuit", max_k: int, tb: int = 13, exact: bool = F
         _check(lib().qsv_window_plan(g.ctypes.data_as(C.c_void_p), g.shape[0], mp, max_k, int(exact), tb,
                                      circ.n, *outs))
     else:
-        width, branch, first, delay, tries = (tuple(int(v) for v in beam) + (0,))[:5]
+        width, branch, first, delay, tries, relabel = (tuple(int(v) for v in beam) + (0, 0))[:6]
         _check(lib().qsv_window_plan_beam(g.ctypes.data_as(C.c_void_p), g.shape[0], mp, max_k, int(exact), tb,
-                                          circ.n, width, branch, first, delay, tries, *outs))
+                                          circ.n, width, branch, first, delay, tries, relabel, *outs))
     out = []
     for b in range(nb.value):
         k = int(ks[b])

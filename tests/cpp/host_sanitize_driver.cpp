@@ -16,6 +16,10 @@
 #include "QRunes.h"
 #include "qsv.h"
 
+namespace qsv {
+std::vector<std::vector<std::pair<int, int>>> qubit_routes(std::vector<int> dest, int tb);
+}
+
 namespace {
 
 struct Rng {
@@ -134,7 +138,7 @@ void planners(Rng &r, int rounds)
                 // the beam search over windows (the learned schedule's planner)
                 const int width = 1 + r.below(4), branch = 1 + r.below(3), first = r.below(7), delay = r.below(5);
                 const int rb = qsv_window_plan_beam(g.data(), count, mats.empty() ? nullptr : mats.data(), k, 0, tb,
-                                                    n, width, branch, first, delay, r.below(9), cap, kq.data(),
+                                                    n, width, branch, first, delay, r.below(9), r.below(2), cap, kq.data(),
                                                     qs.data(), m.data(), ng.data(), win.data(), &nb);
                 check(rb == QSV_OK, "qsv_window_plan_beam");
                 total = 0;
@@ -214,6 +218,52 @@ void qrunes(Rng &r, const std::vector<std::string> &files, int rounds)
 
 }  // namespace
 
+// qubit_routes: random permutations of 4..40 positions; every pass must move
+// a permutation of at most 13 positions that holds 0..3, and the passes must
+// take every position's data to its destination, within the stated bound.
+void routes(Rng &r, int rounds)
+{
+    for (int t = 0; t < rounds * 5; t++) {
+        const int n = 4 + r.below(37);
+        std::vector<int> dest(n);
+        for (int i = 0; i < n; i++) dest[i] = i;
+        const int swaps = r.below(2 * n);
+        for (int i = 0; i < swaps; i++) std::swap(dest[r.below(n)], dest[r.below(n)]);
+        auto passes = qsv::qubit_routes(dest, 13);
+        std::vector<int> data(n);      // data[p] = the original position of the data now at p
+        for (int i = 0; i < n; i++) data[i] = i;
+        bool ok = true;
+        for (const auto &moves : passes) {
+            uint64_t from = 0, to = 0;
+            for (const auto &m : moves) {
+                ok &= m.first >= 0 && m.first < n && m.second >= 0 && m.second < n && m.first != m.second;
+                if (!ok) break;
+                ok &= !(from >> m.first & 1) && !(to >> m.second & 1);
+                from |= uint64_t(1) << m.first;
+                to |= uint64_t(1) << m.second;
+            }
+            ok &= from == to && __builtin_popcountll(from | 15) <= 13 && !moves.empty();
+            if (!ok) break;
+            std::vector<int> nd(data);
+            for (const auto &m : moves) nd[m.second] = data[m.first];
+            data.swap(nd);
+        }
+        for (int p = 0; ok && p < n; p++) ok &= dest[data[p]] == p;
+        // cycles: ceil((c - 1) / 12) passes for the longest cycle outside 0..3 at least,
+        // and never more than the cycles' sum of ceil(c / 9) + 1
+        int bound = 1;
+        std::vector<char> seen(n);
+        for (int p = 0; p < n; p++) {
+            if (seen[p]) continue;
+            int c = 0;
+            for (int q = p; !seen[q]; q = dest[q]) seen[q] = 1, c++;
+            if (c > 1) bound += (c + 8) / 9;
+        }
+        ok &= (int)passes.size() <= bound;
+        check(ok, "qubit routes");
+    }
+}
+
 int main(int argc, char **argv)
 {
     Rng r{0x9E3779B97F4A7C15ull};
@@ -221,6 +271,7 @@ int main(int argc, char **argv)
     std::vector<std::string> files;
     for (int i = 2; i < argc; i++) files.push_back(argv[i]);
     planners(r, rounds);
+    routes(r, rounds);
     qrunes(r, files, rounds * 20);
     std::printf("host sanitize driver: %d planner rounds, %d QRunes texts, %d failures\n", rounds, rounds * 20,
                 g_failures);

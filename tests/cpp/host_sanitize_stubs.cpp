@@ -36,7 +36,7 @@ void fused_operands(int, const double *, int, float *) { unreachable("fused_oper
 int launch_block(qsv_state *, const int *, int, const double *, uint64_t) { return unreachable("launch_block"); }
 int launch_fused_dev(qsv_state *, const int *, int, const float *, int, int) { return unreachable("launch_fused_dev"); }
 int launch_pass_dev(qsv_state *, const std::vector<const std::vector<int> *> &, const std::vector<const float *> &, int,
-                    int, int)
+                    int, int, const std::vector<std::pair<int, int>> *)
 {
     return unreachable("launch_pass_dev");
 }

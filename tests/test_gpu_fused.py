@@ -234,10 +234,14 @@ def test_window_scheduler_vs_oracle(pass_tb, k, split):
         st = sv.run(circ, fuse_k=k, split=split, scheduler=3, pass_tb=pass_tb, profile=True)
         got = sv.get()
     assert st.planner == 1 and st.fuse_k == k
-    windows = len({w for *_, w in E.window_plan(circ, k, pass_tb or 13)})
+    # 13-bit tiles relabel qubits between sweeps (and restore the order in at
+    # most ceil(17 / 12) + 1 sweeps at the end)
+    relabel = pass_tb in (0, 13)
+    beam = (1, 3, 6, 4, 0, 1) if relabel else None
+    windows = len({w for *_, w in E.window_plan(circ, k, pass_tb or 13, beam=beam)})
     cl = st.classes()
     sweeps = cl.get("fused", {}).get("launches", 0) + cl.get("pass", {}).get("launches", 0)
-    assert windows <= sweeps <= windows + st.blocks // 16
+    assert windows <= sweeps <= windows + st.blocks // 16 + (3 if relabel else 0)
     fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
     say("sweeps", sweeps, "blocks", st.blocks, "fidelity", fid)
     assert fid >= 1 - 1e-6
@@ -286,3 +290,34 @@ def test_window_and_learned_schedulers_at_small_sizes(n):
         assert (st.planner == 1) == (sched == 3 and n >= 11) or sched == 1
         assert fid >= 1 - 1e-6
         assert np.abs(got - ref.psi).max() < 1e-5
+
+
+@pytest.mark.parametrize("n", [14, 16, 21])
+@pytest.mark.parametrize("sched", [3, 1])
+def test_relabelled_schedules_vs_oracle(n, sched):
+    """Window schedules with relabelling (each sweep leaves four of its qubits
+    at bits 0..3 for the next window; the order is restored at the end)
+    against the FP64 oracle, and against the same schedule with qubits 0..3
+    fixed: fewer sweeps, the same state to the fused-path bar."""
+    circ = C.random_layers(n, 30, 7300 + n, haar_frac=0.1)
+    ref, _ = O.run_circuit(circ)
+    m = E.CostModel.load() if os.path.exists(E.DEFAULT_COSTMODEL) else E.CostModel(5)
+    sweeps, states = {}, {}
+    for fixed in (False, True):
+        with E.StateVector(n, E.C64) as sv:
+            if sched == 1:
+                sv.attach_costmodel(m)
+            st = sv.run(circ, fuse_k=4, split=3, scheduler=sched, profile=True, fixed_low=fixed, rng_state=5)
+            got = sv.get()
+        cl = st.classes()
+        sweeps[fixed] = cl.get("fused", {}).get("launches", 0) + cl.get("pass", {}).get("launches", 0)
+        states[fixed] = got
+        fid = abs(np.vdot(ref.psi, got)) ** 2 / np.vdot(got, got).real
+        say("n", n, "scheduler", sched, "fixed_low", fixed, "sweeps", sweeps[fixed], "blocks", st.blocks,
+            "fidelity", fid)
+        assert fid >= 1 - 1e-6
+        assert np.abs(got - ref.psi).max() < 1e-5
+    # at 14 qubits a window holds all but one qubit: no fewer windows, and the
+    # last window's store puts the qubits back
+    assert sweeps[False] <= sweeps[True] if n < 18 else sweeps[False] < sweeps[True]
+    assert np.abs(states[False] - states[True]).max() < 1e-5
