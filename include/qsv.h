@@ -432,9 +432,10 @@ int qsv_window_plan(const qsv_gate *gates, int64_t ngates, const double *mats, i
  * orders.  relabel != 0: each window grows from the four qubits the previous
  * one leaves at bits 0..3 (see qsv_run_opts.fixed_low) instead of from qubits
  * 0..3; the blocks keep their logical qubits.  width = 1, fuse_tries = 0 and
- * relabel = 0 give qsv_window_plan's plan.  Outputs as qsv_window_plan.  The learned scheduler (opts.scheduler = 1) plans this
- * way while the first windows run: width 12, branch 4, first 8, delay 6 for
- * >= 30 qubits, width 4, branch 3, first 6, delay 4 at 29, width 1 below;
+ * relabel = 0 give qsv_window_plan's plan.  Outputs as qsv_window_plan.  The
+ * learned scheduler (opts.scheduler = 1) plans this way while the first
+ * windows run, relabelled from 14 qubits: width 8, branch 4, first 8, delay 6
+ * for >= 30 qubits, width 4, branch 3, first 6, delay 4 at 29, width 1 below;
  * 24 fusion tries from 29 qubits, 8 at 28, none below.  No reference
  * counterpart. */
 int qsv_window_plan_beam(const qsv_gate *gates, int64_t ngates, const double *mats, int max_k, int exact,

@@ -756,12 +756,13 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
         send_error(fail(QSV_ERR_DEVICE, "cannot select the device"));
         return;
     }
-    // the beam search costs host time per window (~0.5 ms at width 4, ~1.5 ms
-    // at width 12 on 30 qubits): it is sized to stay ahead of the device,
-    // whose window sweeps take ~2 ms at 29 qubits and ~4.2 ms at 30
+    // the beam search costs host time per window (~0.5 ms at width 4 on 29
+    // qubits; with relabelling ~2.8 ms at width 8 on 30, ~4.4 ms at width 12
+    // for the same windows): it is sized to stay ahead of the device, whose
+    // window sweeps take ~2 ms at 29 qubits and ~4.7 ms at 30
     BeamOpts bo;
     if (learned && s->n >= 30) {
-        bo.width = 12;
+        bo.width = 8;
         bo.branch = 4;
         bo.first = 8;
         bo.delay = 6;

@@ -562,13 +562,26 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
                 soff |= pa.st_hbit[j];
                 sl ^= pa.st_lbit[j];
             }
+        if (pa.st_pair == 1) {
+            // qubit 0 stays: the pair is adjacent and 8-byte aligned
 #pragma unroll
-        for (int i = 0; i < NLD; i++) {
-            const int e0 = sl ^ pa.st_li[i], e1 = e0 ^ pa.st_pair;
-            float4 v;
-            v.x = pre[e0] * li; v.y = pim[e0] * li;
-            v.z = pre[e1] * li; v.w = pim[e1] * li;
-            *reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]) = v;
+            for (int i = 0; i < NLD; i++) {
+                const int e = sl ^ pa.st_li[i];
+                const float2 r = *reinterpret_cast<const float2 *>(pre + e), m = *reinterpret_cast<const float2 *>(pim + e);
+                float4 v;
+                v.x = r.x * li; v.y = m.x * li;
+                v.z = r.y * li; v.w = m.y * li;
+                *reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]) = v;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NLD; i++) {
+                const int e0 = sl ^ pa.st_li[i], e1 = e0 ^ pa.st_pair;
+                float4 v;
+                v.x = pre[e0] * li; v.y = pim[e0] * li;
+                v.z = pre[e1] * li; v.w = pim[e1] * li;
+                *reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]) = v;
+            }
         }
     } else {
 #pragma unroll

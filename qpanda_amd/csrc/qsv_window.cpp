@@ -573,7 +573,9 @@ int plan_window(const qsv_gate *gates, int64_t ng, const double *mats, int max_k
 // expansions of a depth run on a few worker threads.  On the 30-qubit
 // depth-200 benchmark circuit: 234 windows with width 1, 221 with width 4,
 // branch 3, first 6, delay 4, 216 with width 12, branch 4, first 8, delay 6
-// (another seed: 238, 223, 219).
+// (another seed: 238, 223, 219); relabelled (each window grows from the four
+// qubits the previous one left at bits 0..3): 188 with width 1, 180 with
+// width 4, 176 with width 8 or 12 (another seed: 177).
 struct WindowStream::Impl {
     const qsv_gate *gates = nullptr;
     int64_t ng = 0;
