@@ -908,26 +908,35 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
                 L.tb = kMaxPassTileBits;
                 L.tc = 0;
             }
-        } else if (bo.relabel && u.next_low != u.low) {
-            // next_low's qubits move into the positions 0..3 whose qubits leave
+        } else if (bo.relabel) {
+            // next_low's qubits move into the positions 0..3 whose qubits
+            // leave; then, among the tile's other positions, every qubit whose
+            // own bit is one of them goes there (free in the same store, and
+            // the order to restore at the end stays short)
+            std::vector<int> nat(at);
             int slot[4], ns = 0;
             for (int p = 0; p < 4; p++)
                 if (!(u.next_low >> at[p] & 1)) slot[ns++] = p;
-            Launch &L = w.launches.back();
             int si = 0;
             for (uint64_t m = u.next_low; m; m &= m - 1) {
                 const int x = __builtin_ctzll(m);
                 if (phys[x] < 4) continue;
-                const int j = slot[si++], p = phys[x], y = at[j];
-                L.moves.push_back({p, j});
-                L.moves.push_back({j, p});
-                at[j] = x;
-                at[p] = y;
-                phys[x] = j;
-                phys[y] = p;
+                const int j = slot[si++], p = phys[x];
+                std::swap(nat[j], nat[p]);
             }
-            L.tb = kMaxPassTileBits;
-            L.tc = 0;
+            uint64_t tile = 0;
+            for (uint64_t m = u.mask; m; m &= m - 1) tile |= uint64_t(1) << phys[__builtin_ctzll(m)];
+            const std::vector<int> mid(nat);
+            for (const auto &mv : route_within(mid, tile & ~uint64_t(15))) nat[mv.second] = mid[mv.first];
+            Launch &L = w.launches.back();
+            for (int p = 0; p < n; p++)
+                if (nat[p] != at[p]) L.moves.push_back({phys[nat[p]], p});
+            at.swap(nat);
+            for (int p = 0; p < n; p++) phys[at[p]] = p;
+            if (!L.moves.empty()) {
+                L.tb = kMaxPassTileBits;
+                L.tc = 0;
+            }
         }
         w.k = k;
         w.pred = best;
