@@ -345,8 +345,9 @@ typedef struct qsv_run_opts {
     int pass_tb;        /* scheduler 2: tile bits of such a pass (0 = minimal)  */
     int pass_kernel;    /* multi-block sweeps: 0 or 1 = one workgroup per tile  */
                         /* (other values: QSV_ERR_UNSUPPORTED)                  */
-    int fixed_low;      /* window schedules: 1 = qubits 0..3 stay in place (no  */
-                        /* relabelling between sweeps, see below)               */
+    int fixed_low;      /* window schedules: 0 = relabel qubits between sweeps  */
+                        /* from 28 qubits, 1 = never (qubits 0..3 stay in      */
+                        /* place), -1 = from 14 qubits (see below)              */
 } qsv_run_opts;
 /* opts.scheduler: 0 = greedy blocks at fuse_k, one sweep each;
  *                 1 = learned (planner, block size, grouping into sweeps,
@@ -357,8 +358,8 @@ typedef struct qsv_run_opts {
  *                     inside it fused into blocks of <= fuse_k (4, 5)
  *                     qubits, one sweep per window (states of >= 11 qubits;
  *                     smaller ones fall back to scheduler 0).
- * Relabelling (schedulers 1 and 3, >= 14 qubits, 13-bit tiles, unless
- * fixed_low): a sweep's tile always holds the amplitudes' bits 0..3 (128-byte
+ * Relabelling (schedulers 1 and 3, 13-bit tiles; >= 28 qubits, or >= 14 with
+ * fixed_low = -1): a sweep's tile always holds the amplitudes' bits 0..3 (128-byte
  * runs), but which qubits sit there changes: each window's sweep stores its
  * tile with four of the window's qubits moved to bits 0..3 (those the next
  * window starts from), so consecutive windows share four qubits instead of

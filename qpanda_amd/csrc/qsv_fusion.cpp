@@ -682,6 +682,9 @@ int exec_launches(qsv_state *s, const qsv_gate *gates, const std::vector<PlanIte
 // k = 5 only on a predicted gain above the model's per-launch error (at 5%
 // the model picked k = 5 for windows that ran slower)
 constexpr double kK5Margin = 0.10;
+// smallest state the window schedules relabel by default (held-out circuits
+// at 26 qubits ran 0.80-1.06x as fast relabelled, at 28 qubits 1.04-1.06x)
+constexpr int kRelabelFrom = 28;
 // random dependency orders the learned schedule's fusion tries per window
 // (~12 us each at k = 4 and 5 on ~40 records): 24 where a window's sweep
 // takes >= 2 ms (>= 29 qubits), 8 at 28 qubits (~1 ms), none below, so that
@@ -780,7 +783,11 @@ void produce_windows(qsv_state *s, const qsv_gate *gates, int64_t ngates, const 
     // RESET and at the end, sweeps that only move qubits bring every qubit
     // back to its own position.
     const int n = s->n;
-    bo.relabel = !o.fixed_low && n > kMaxPassTileBits && (o.pass_tb == 0 || o.pass_tb == kMaxPassTileBits);
+    // (by default from 28 qubits: below, a window sweep takes < 1 ms and the
+    // planning thread, which relabelling gives one more growth per window,
+    // falls behind the device; fixed_low = -1 relabels from 14 qubits)
+    const int relabel_from = o.fixed_low < 0 ? kMaxPassTileBits + 1 : kRelabelFrom;
+    bo.relabel = o.fixed_low <= 0 && n >= relabel_from && (o.pass_tb == 0 || o.pass_tb == kMaxPassTileBits);
     std::vector<int> phys(n), at(n);
     for (int x = 0; x < n; x++) phys[x] = at[x] = x;
     auto restore = [&]() {

@@ -375,7 +375,8 @@ class StateVector:
         cost model), 2 greedy blocks grouped into multi-block sweeps of <= group,
         3 window planner (one sweep per window of pass_tb tile bits, 0 = 13).
         pass_kernel: 0 or 1 (one workgroup per tile).  fixed_low: window
-        schedules keep qubits 0..3 in every sweep (no relabelling)."""
+        schedules keep qubits 0..3 in every sweep (True), relabel qubits
+        between sweeps from 28 qubits (False) or from 14 qubits (-1)."""
         g = np.ascontiguousarray(circ.gates)
         m = np.ascontiguousarray(circ.mats, np.complex128)
         opts = RunOpts(int(exact), int(fuse_k), int(split), int(scheduler), int(profile), int(tile),
