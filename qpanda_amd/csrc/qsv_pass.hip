@@ -45,6 +45,7 @@
 namespace qsv {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // Threads per workgroup: 512 (8 waves) for 64 KiB tiles of k = 4 blocks, so
@@ -516,7 +517,13 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
         if ((tid >> (j - 1)) & 1) toff |= uint64_t(1) << pa.lpos[j];
     float4 buf[NLD];
 #pragma unroll
-    for (int i = 0; i < NLD; i++) buf[i] = *reinterpret_cast<const float4 *>(a + base + toff + pa.hioff[i]);
+    // nontemporal: each amplitude is read once per sweep (1% faster per
+    // benchmark sweep than cached loads; nontemporal stores alone changed
+    // nothing)
+    for (int i = 0; i < NLD; i++) {
+        const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + base + toff + pa.hioff[i]));
+        buf[i] = make_float4(t.x, t.y, t.z, t.w);
+    }
     // elements 2 tid + 2 NT i: the swizzle is linear, so it is one XOR for
     // tid's bits and a constant per (unrolled) i; it keeps bit 0, so element +
     // 1 follows at index + 1
@@ -571,7 +578,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
                 float4 v;
                 v.x = r.x * li; v.y = m.x * li;
                 v.z = r.y * li; v.w = m.y * li;
-                *reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]) = v;
+                __builtin_nontemporal_store((f4v){v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(a + base + soff + pa.st_hioff[i]));
             }
         } else {
 #pragma unroll
@@ -580,7 +587,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
                 float4 v;
                 v.x = pre[e0] * li; v.y = pim[e0] * li;
                 v.z = pre[e1] * li; v.w = pim[e1] * li;
-                *reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]) = v;
+                __builtin_nontemporal_store((f4v){v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(a + base + soff + pa.st_hioff[i]));
             }
         }
     } else {
@@ -591,7 +598,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
             float4 v;
             v.x = r.x * li; v.y = m.x * li;
             v.z = r.y * li; v.w = m.y * li;
-            *reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]) = v;
+            __builtin_nontemporal_store((f4v){v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(a + base + toff + pa.hioff[i]));
         }
     }
 }
