@@ -506,11 +506,12 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     float *const pre = planes[0];
     float *const pim = planes[1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // workgroup b runs on XCD b % 8 (round-robin dispatch): each XCD takes a
-    // contiguous eighth of the tiles (1.5% faster per benchmark sweep than
-    // tile = b, bit-identical)
-    const uint64_t per = ntiles >> 3;
-    const uint64_t tile = (ntiles & 7) ? blockIdx.x : (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    // workgroup b runs on XCD b % 8 (round-robin dispatch): each XCD takes
+    // runs of 64 consecutive tiles (benchmark: 2.2% faster than tile = b; runs
+    // of 8, 16, 32, 1024 tiles and a contiguous eighth per XCD measured
+    // 0.4-1.5% slower than 64; bit-identical)
+    const uint64_t q = blockIdx.x >> 3;
+    const uint64_t tile = (ntiles & 511) ? blockIdx.x : (q >> 6) * 512 + (blockIdx.x & 7) * 64 + (q & 63);
     if (tile >= ntiles) return;
     const uint64_t base = expand_index(tile, pa.tile_ins);
     // element 2 tid + 2 NT i sits at base + pdep(2 tid, L) + pdep(2 NT i, L);
