@@ -54,6 +54,7 @@
 // (i/8)*4 + i%4.  32x32x16 bf16: A[i][k] in lane i + 32*(k/8), element k%8;
 // B[k][j] in lane j + 32*(k/8), element k%8; same D layout.
 #include "qsv_internal.h"
+#include "qsv_stream.h"
 #include "qsv_mfma.h"
 #include "qsv_sched.h"
 
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     float *const pim = planes[1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t tile = blockIdx.x;
+    const uint64_t tile = xcd_block(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const uint64_t base = expand_index(tile, fa.tile_ins);
 
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
     for (int i = 0; i < NLD; i++) {
         const int e = 2 * (tid + THREADS * i);
         off[i] = base + (uint64_t)(e & (RUN - 1)) + fa.off_high[e >> TC];
-        buf[i] = *reinterpret_cast<const float4 *>(a + off[i]);
+        buf[i] = ld_nt(reinterpret_cast<const float4 *>(a + off[i]));
     }
 
     // A operands (the matrix), operand-ready layout: mops[(h*KSTEPS + s)*64 + lane]
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(2 << TC) void k_fused_mfma(float2 *__restrict__ a, 
         float4 v;
         v.x = r.x * li; v.y = m.x * li;
         v.z = r.y * li; v.w = m.y * li;
-        *reinterpret_cast<float4 *>(a + off[i]) = v;
+        st_nt(reinterpret_cast<float4 *>(a + off[i]), v);
     }
 }
 

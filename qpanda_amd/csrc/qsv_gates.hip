@@ -17,6 +17,7 @@
 // in flight per thread.  The sweeps are HBM-bound: 2 x 16 B read + written per
 // pair for c128.
 #include "qsv_gatemath.h"
+#include "qsv_stream.h"
 
 namespace qsv {
 
@@ -24,46 +25,13 @@ namespace {
 
 constexpr int kPairsPerThread = 4;
 
-// Nontemporal amplitude accesses (each amplitude is read and written once per
-// sweep) and the XCD block mapping: block b runs on XCD b % 8, so each XCD
-// takes runs of 64 consecutive blocks of the index range.  At 30 qubits
-// (c128) the mapping alone took an uncontrolled 2x2 sweep from 6.40 to 6.30
-// ms, and with nontemporal accesses to 6.11 ms (5.6 TB/s); sweeps that touch
-// a subset of the amplitudes (controlled gates, iSWAP: partial cache lines)
-// ran 9% slower with nontemporal accesses, so only the dense pair sweep uses
-// them.
-template <typename T> struct NtVec;
-template <> struct NtVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
-template <> struct NtVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
-template <typename T>
-__device__ __forceinline__ T ld_nt(const T *p)
-{
-    const typename NtVec<T>::type v = __builtin_nontemporal_load(reinterpret_cast<const typename NtVec<T>::type *>(p));
-    T r;
-    r.x = v.x;
-    r.y = v.y;
-    return r;
-}
-template <typename T>
-__device__ __forceinline__ void st_nt(T *p, T x)
-{
-    const typename NtVec<T>::type v = {x.x, x.y};
-    __builtin_nontemporal_store(v, reinterpret_cast<typename NtVec<T>::type *>(p));
-}
-__device__ __forceinline__ uint64_t xcd_block()
-{
-    const uint64_t b = blockIdx.x;
-    if (gridDim.x & 511) return b;
-    const uint64_t q = b >> 3;
-    return (q >> 6) * 512 + (b & 7) * 64 + (q & 63);
-}
 
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_pair(T *__restrict__ a, uint64_t npairs, InsertSet ins,
                                                   uint64_t or0, uint64_t or1,
                                                   Coef<typename Real<T>::type> c)
 {
-    const uint64_t base = xcd_block() * (kBlock * kPairsPerThread) + threadIdx.x;
+    const uint64_t base = xcd_block(blockIdx.x, gridDim.x) * (kBlock * kPairsPerThread) + threadIdx.x;
     T x0[kPairsPerThread], x1[kPairsPerThread];
     const bool dense = ins.n == 1 && or0 == 0;     // every amplitude in some pair
     uint64_t i0[kPairsPerThread];
@@ -105,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(T *__restrict__ a, uint64_t co
                                                    uint64_t orm, typename Real<T>::type re,
                                                    typename Real<T>::type im)
 {
-    const uint64_t base = xcd_block() * (kBlock * kPairsPerThread) + threadIdx.x;
+    const uint64_t base = xcd_block(blockIdx.x, gridDim.x) * (kBlock * kPairsPerThread) + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < kPairsPerThread; k++) {
         const uint64_t p = base + (uint64_t)k * kBlock;

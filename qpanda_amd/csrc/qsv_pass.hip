@@ -41,11 +41,11 @@
 #include "qsv_internal.h"
 #include "qsv_mfma.h"
 #include "qsv_sched.h"
+#include "qsv_stream.h"
 
 namespace qsv {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
-typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // Threads per workgroup: 512 (8 waves) for 64 KiB tiles of k = 4 blocks, so
@@ -506,12 +506,10 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     float *const pre = planes[0];
     float *const pim = planes[1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // workgroup b runs on XCD b % 8 (round-robin dispatch): each XCD takes
-    // runs of 64 consecutive tiles (benchmark: 2.2% faster than tile = b; runs
-    // of 8, 16, 32, 1024 tiles and a contiguous eighth per XCD measured
-    // 0.4-1.5% slower than 64; bit-identical)
-    const uint64_t q = blockIdx.x >> 3;
-    const uint64_t tile = (ntiles & 511) ? blockIdx.x : (q >> 6) * 512 + (blockIdx.x & 7) * 64 + (q & 63);
+    // XCD mapping (benchmark: 2.2% faster than tile = b; runs of 8, 16, 32,
+    // 128, 256, 1024 tiles and a contiguous eighth per XCD measured 0.4-1.5%
+    // slower than runs of 64; bit-identical)
+    const uint64_t tile = xcd_block(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const uint64_t base = expand_index(tile, pa.tile_ins);
     // element 2 tid + 2 NT i sits at base + pdep(2 tid, L) + pdep(2 NT i, L);
@@ -525,10 +523,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
     // per sweep (with the XCD tile mapping, 2% faster per benchmark circuit
     // than nontemporal loads with cached stores)
 #pragma unroll
-    for (int i = 0; i < NLD; i++) {
-        const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + base + toff + pa.hioff[i]));
-        buf[i] = make_float4(t.x, t.y, t.z, t.w);
-    }
+    for (int i = 0; i < NLD; i++) buf[i] = ld_nt(reinterpret_cast<const float4 *>(a + base + toff + pa.hioff[i]));
     // elements 2 tid + 2 NT i: the swizzle is linear, so it is one XOR for
     // tid's bits and a constant per (unrolled) i; it keeps bit 0, so element +
     // 1 follows at index + 1
@@ -583,7 +578,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
                 float4 v;
                 v.x = r.x * li; v.y = m.x * li;
                 v.z = r.y * li; v.w = m.y * li;
-                __builtin_nontemporal_store((f4v){v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(a + base + soff + pa.st_hioff[i]));
+                st_nt(reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]), v);
             }
         } else {
 #pragma unroll
@@ -592,7 +587,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
                 float4 v;
                 v.x = pre[e0] * li; v.y = pim[e0] * li;
                 v.z = pre[e1] * li; v.w = pim[e1] * li;
-                __builtin_nontemporal_store((f4v){v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(a + base + soff + pa.st_hioff[i]));
+                st_nt(reinterpret_cast<float4 *>(a + base + soff + pa.st_hioff[i]), v);
             }
         }
     } else {
@@ -603,7 +598,7 @@ __device__ __forceinline__ void fused_pass_body(float2 *__restrict__ a, uint64_t
             float4 v;
             v.x = r.x * li; v.y = m.x * li;
             v.z = r.y * li; v.w = m.y * li;
-            __builtin_nontemporal_store((f4v){v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(a + base + toff + pa.hioff[i]));
+            st_nt(reinterpret_cast<float4 *>(a + base + toff + pa.hioff[i]), v);
         }
     }
 }

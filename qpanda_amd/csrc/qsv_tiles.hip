@@ -17,6 +17,7 @@
 // sweep the state once per gate (CPUQuantumGates.cpp:92-1435,
 // GPUGates.cu:129-1020).
 #include "qsv_gatemath.h"
+#include "qsv_stream.h"
 
 #include <algorithm>
 #include <vector>
@@ -98,13 +99,13 @@ __global__ __launch_bounds__(kTileThreads) void k_gate_tile(T *__restrict__ a, T
     static_assert(PER >= 1 && PER <= 16, "tile size and thread count");
     __shared__ T t[TS];
     const int tid = threadIdx.x;
-    const uint64_t base = expand_index(blockIdx.x, ta.tile_ins);
+    const uint64_t base = expand_index(xcd_block(blockIdx.x, gridDim.x), ta.tile_ins);
     uint64_t toff = 0;
     for (int j = 0; j < TB && (1 << j) < kTileThreads; j++)
         if ((tid >> j) & 1) toff |= uint64_t(1) << ta.lpos[j];
     T v[PER];
 #pragma unroll
-    for (int i = 0; i < PER; i++) v[i] = a[base + toff + ta.hioff[i]];
+    for (int i = 0; i < PER; i++) v[i] = ld_nt(a + base + toff + ta.hioff[i]);
 #pragma unroll
     for (int i = 0; i < PER; i++) t[tid + kTileThreads * i] = v[i];
     __syncthreads();
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gate_tile(T *__restrict__ a, T
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < PER; i++) a[base + toff + ta.hioff[i]] = t[tid + kTileThreads * i];
+    for (int i = 0; i < PER; i++) st_nt(a + base + toff + ta.hioff[i], t[tid + kTileThreads * i]);
 }
 
 uint32_t pext(uint64_t x, const int *lpos, int tb)
