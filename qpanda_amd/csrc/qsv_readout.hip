@@ -10,6 +10,7 @@
 // (CPUQuantumGates.cpp:1077-1083), which serial_prob0 reproduces on the host
 // (serial_prob0_host, qsv_io.cpp) for the measurement tie band.
 #include "qsv_internal.h"
+#include "qsv_stream.h"
 
 #include <algorithm>
 #include <cstring>
@@ -55,7 +56,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const T *__restrict__ a, uint
                                                     InsertSet ins, int nq, double *__restrict__ partial)
 {
     __shared__ double lds[kBlock / 64];
-    const uint64_t start = (uint64_t)blockIdx.x * chunk;
+    // chunk c of the partition, wherever its workgroup runs (XCD mapping;
+    // the sums and their order do not change)
+    const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t start = blk * chunk;
     uint64_t end = start + chunk;
     if (end > count) end = count;
     const uint64_t zmask = nq > 1 ? ((uint64_t(1) << (nq - 1)) - 1) : 0;
@@ -65,13 +69,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const T *__restrict__ a, uint
             acc += norm_of(a[expand_index(i, ins)]);
         } else if (MODE == RED_ZZ) {
             const int flips = __popcll((i ^ (i >> 1)) & zmask);
-            acc += norm_of(a[i]) * (double)(nq - 1 - 2 * flips);
+            acc += norm_of(ld_nt(a + i)) * (double)(nq - 1 - 2 * flips);
         } else {
-            acc += norm_of(a[i]);
+            acc += norm_of(ld_nt(a + i));
         }
     }
     const double t = block_sum(acc, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (threadIdx.x == 0) partial[blk] = t;
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ partial, int n,
