@@ -94,16 +94,16 @@ __global__ __launch_bounds__(kBlock) void k_collapse(T *__restrict__ a, uint64_t
 {
     // grid-stride: a launch stays below 2^32 threads at any state size
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npairs; p += stride) {
+    for (uint64_t p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; p < npairs; p += stride) {
         const uint64_t i0 = expand_index(p, ins), i1 = i0 | bit;
         const uint64_t keep = outcome ? i1 : i0, drop = outcome ? i0 : i1;
         using R = decltype(T{}.x);
-        T v = a[keep], z;
+        T v = ld_nt(a + keep), z;
         z.x = 0; z.y = 0;
         v.x = (R)((double)v.x * scale);
         v.y = (R)((double)v.y * scale);
-        a[keep] = v;
-        a[drop] = z;
+        st_nt(a + keep, v);
+        st_nt(a + drop, z);
     }
 }
 
@@ -114,12 +114,15 @@ __global__ __launch_bounds__(kBlock) void k_inner(const TA *__restrict__ a, cons
                                                    uint64_t count, uint64_t chunk, double *__restrict__ partial)
 {
     __shared__ double lds[kBlock / 64];
-    const uint64_t start = (uint64_t)blockIdx.x * chunk;
+    const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t start = blk * chunk;
     uint64_t end = start + chunk;
     if (end > count) end = count;
     double re = 0, im = 0;
     for (uint64_t i = start + threadIdx.x; i < end; i += kBlock) {
-        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        const TA va = ld_nt(a + i);
+        const TB vb = ld_nt(b + i);
+        const double ar = va.x, ai = va.y, br = vb.x, bi = vb.y;
         re += ar * br + ai * bi;          // conj(a) * b
         im += ar * bi - ai * br;
     }
@@ -127,8 +130,8 @@ __global__ __launch_bounds__(kBlock) void k_inner(const TA *__restrict__ a, cons
     __syncthreads();
     const double ti = block_sum(im, lds);
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = tr;
-        partial[2 * blockIdx.x + 1] = ti;
+        partial[2 * blk] = tr;
+        partial[2 * blk + 1] = ti;
     }
 }
 
@@ -180,8 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_marg_tile(const T *__restrict__ a, i
 {
     __shared__ double p[1 << kTileBits];
     const int tsize = 1 << tb;
-    const uint64_t base = (uint64_t)blockIdx.x << tb;
-    for (int i = threadIdx.x; i < tsize; i += kBlock) p[i] = norm_of(a[base + i]);
+    const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t base = blk << tb;
+    for (int i = threadIdx.x; i < tsize; i += kBlock) p[i] = norm_of(ld_nt(a + base + i));
     __syncthreads();
     int done = 0;
     for (int u = 0; u < tb; u++) {
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_marg_tile(const T *__restrict__ a, i
         int idx = 0;
         for (int r = 0; r < ml; r++)
             if (lb >> r & 1) idx |= 1 << low_pos.pos[r];
-        partial[(uint64_t)blockIdx.x * nb + lb] = p[idx];
+        partial[blk * nb + lb] = p[idx];
     }
 }
 

@@ -6,7 +6,9 @@
 // consecutive blocks of the index range (grids that are not a multiple of 512
 // keep b).  Measured at 30 qubits, bit-identical: window sweeps 2-5% faster,
 // the uncontrolled 2x2 gate sweep 6.40 -> 6.11 ms, gate-tiled sweeps 4.5%,
-// single-block fused sweeps 6%; sweeps that touch only part of each cache
+// single-block fused sweeps 6%, the FP64 norm of a c128 state 2.88 -> 2.46
+// ms, <a|b> 11-13%, marginals 7%, collapse 10% (the reductions keep their
+// partition and summation order); sweeps that touch only part of each cache
 // line (controlled gates) ran slower with nontemporal accesses and keep
 // cached ones.
 #pragma once
